@@ -1,0 +1,165 @@
+/*
+ * mpf.h -- C ABI of the MI355X multiphase-field (MPF) time step.
+ *
+ * This is the boundary behind which OpenPhase's single-resolution grain-growth step runs on one
+ * MI355X (gfx950). Each entry point replaces one reference call (cited below); the C++ classes in
+ * include/openphase_amd/OpenPhaseAMD.h keep the reference class/method names on top of it.
+ *
+ * State: per cell a fixed-capacity, insertion-ordered list of at most `slots` (phase-field index,
+ * value) entries, stored slot-major (structure of arrays) in device memory owned by the context.
+ * Host buffers are owned by the caller. Calls on one context are not thread-safe; all device work
+ * is ordered on the context's stream. No entry point ever exits the process: every failure is a
+ * status code, with details from mpf_error_message().
+ */
+#ifndef OPENPHASE_AMD_MPF_H
+#define OPENPHASE_AMD_MPF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mpf_status {
+    MPF_OK = 0,
+    MPF_ERR_ARG = 1,               /* invalid argument (message says which) */
+    MPF_ERR_UNSUPPORTED = 2,       /* configuration outside what the library implements */
+    MPF_ERR_HIP = 3,               /* HIP runtime error */
+    MPF_ERR_NOMEM = 4,             /* device or host allocation failed */
+    MPF_SLOT_OVERFLOW = 5,         /* a cell needed more than `slots` entries; see mpf_overflow_cell */
+    MPF_EMPTY_NODE = 6,            /* finalize erased every entry of a cell (reference: undefined) */
+    MPF_ERR_STATE = 7              /* call out of order (e.g. step before upload) */
+} mpf_status;
+
+typedef enum mpf_precision {
+    MPF_FP64 = 0,                  /* FP64 storage and arithmetic: the reference's arithmetic */
+    MPF_FP32 = 1                   /* FP32 storage, stencils and increments; FP64 limiter,
+                                      normalization, finalize and grain volumes */
+} mpf_precision;
+
+typedef enum mpf_bc {              /* BoundaryConditionTypes, include/BoundaryConditions.h:37-45 */
+    MPF_BC_PERIODIC = 0,
+    MPF_BC_NOFLUX = 1,
+    MPF_BC_FREE = 2,
+    MPF_BC_FIXED = 3,
+    MPF_BC_MIRROR = 4
+} mpf_bc;
+
+typedef enum mpf_tj_model {        /* TripleJunctionModels, src/InterfaceProperties.cpp:274-290 */
+    MPF_TJ_MODEL = 0,
+    MPF_TJ_VALUE = 1,
+    MPF_TJ_NONE = 2
+} mpf_tj_model;
+
+typedef enum mpf_energy_model {    /* InterfaceEnergyModels subset, InterfaceEnergyModel.h:36-49 */
+    MPF_ENERGY_ISO = 0,
+    MPF_ENERGY_CUBIC = 1
+} mpf_energy_model;
+
+/* GridParameters (include/GridParameters.h:37-172) + BoundaryConditions + storage options. */
+typedef struct mpf_grid {
+    int32_t nx, ny, nz;            /* cells per axis; 0 = reduced axis (GridParameters.cpp:645-706) */
+    double dx;                     /* grid spacing [m] */
+    double iwidth;                 /* interface width in cells; Eta = iwidth*dx */
+    int32_t bc[6];                 /* BC0X, BCNX, BC0Y, BCNY, BC0Z, BCNZ (mpf_bc) */
+    int32_t precision;             /* mpf_precision */
+    int32_t slots;                 /* per-cell stored-list capacity K, 2..16 (8 for the benchmarks) */
+} mpf_grid;
+
+/* InterfaceProperties / PhaseField / DrivingForce parameters for Nphases thermodynamic phases.
+ * Tables are nphases*nphases, row-major, indexed [phaseA*nphases + phaseB]. */
+typedef struct mpf_params {
+    int32_t nphases;
+    const double* sigma;           /* Sigma_n_m   interface energy [J/m^2] */
+    const double* mu;              /* Mu_n_m      interface mobility [m^4/(J s)] */
+    const int32_t* energy_model;   /* EnergyModel_n_m (mpf_energy_model), NULL = all ISO */
+    const double* epsilon;         /* EpsilonE_n_m for CUBIC, NULL = 0 */
+    int32_t tj_model;              /* mpf_tj_model */
+    double tj_factor;              /* TripleJunctionFactor */
+    double tj_energy;              /* TripleJunctionEnergy */
+    int32_t consider_nucleus_volume; /* PhaseField ConsiderNucleusVolume (default 1) */
+    double nucleus_volume_factor;  /* PhaseField NucleusVolumeFactor (default 1.0) */
+    const int32_t* dg_mode;        /* UDF_Mode_n_m: 1 = VALUE, 0 = NONE; NULL = no driving force */
+    const double* dg_value;        /* UDF_Value_n_m, antisymmetric [J/m^3] */
+    double dg_limit;               /* DrivingForce Limit_n_m (default 0.95) */
+    double regularization_factor;  /* InterfaceProperties RegularizationFactor (default 1.0) */
+} mpf_params;
+
+typedef struct mpf_ctx mpf_ctx;
+
+/* Construction: PhaseField/InterfaceProperties/BoundaryConditions/DoubleObstacle constructors
+ * (Initialize + ReadInput, include/OPObject.h:64-77). Allocates the device state. */
+mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** out);
+
+/* Grains: PhaseField::AddGrainInfo x n (src/PhaseField.cpp:2970-2979) + per-grain orientation
+ * (GrainsProperties Orientation, rotation matrix row-major 9 doubles, NULL = identity) and
+ * aggregate state (1 = solid, NULL = all solid). Resets Volume/MAXVolume, VolumeRatio = 0. */
+mpf_status mpf_set_grains(mpf_ctx* ctx, int32_t ngrains, const int32_t* phase,
+                          const double* rotation, const uint8_t* solid);
+
+/* PhaseField::Read(FileName) (src/PhaseField.cpp:2487-2543, NodePF::read NodePF.h:751-763):
+ * count[ncells] entries per cell, then the concatenated (index, value) lists, cells ordered
+ * x outer, z inner (the raw .dat order). Host buffers; copied to the device. */
+mpf_status mpf_upload(mpf_ctx* ctx, const int32_t* count, const int32_t* index, const double* value);
+
+/* PhaseField::FinalizeInitialization (src/PhaseField.cpp:1068-1077). */
+mpf_status mpf_finalize_initialization(mpf_ctx* ctx);
+
+/* One full step, nsteps times, at fixed dt, on the device-resident state:
+ *   InterfaceProperties::Set (src/InterfaceProperties.cpp:557-576)
+ *   DoubleObstacle::CalculatePhaseFieldIncrements (src/DoubleObstacle.cpp:461-500)
+ *   [UserDrivingForce::SetDrivingForce + DrivingForce::Average (no averaging) +
+ *    DrivingForce::MergePhaseFieldIncrements when dg_mode is set]
+ *   PhaseField::NormalizeIncrements (src/PhaseField.cpp:1334-1349)
+ *   PhaseField::MergeIncrements + Finalize (src/PhaseField.cpp:1248-1266, 989-1033)
+ * Enqueued on the context's stream; returns after the last step completed and its status was
+ * read back (one synchronisation per call). */
+mpf_status mpf_step(mpf_ctx* ctx, double dt, int32_t nsteps);
+
+/* Same as mpf_step but does not synchronise; errors surface at the next synchronising call. */
+mpf_status mpf_step_async(mpf_ctx* ctx, double dt, int32_t nsteps);
+
+/* Blocks until all work on the context's stream is done; returns the accumulated status. */
+mpf_status mpf_synchronize(mpf_ctx* ctx);
+
+/* Total entries over all cells (size of the index/value download buffers). */
+mpf_status mpf_total_entries(mpf_ctx* ctx, int64_t* total);
+
+/* PhaseField::Write (src/PhaseField.cpp:2340-2388) content: finalized lists in the same layout
+ * as mpf_upload; flag (0/1/2) per cell may be NULL. */
+mpf_status mpf_download(mpf_ctx* ctx, int32_t* count, int32_t* index, double* value, int8_t* flag);
+
+/* GrainsProperties Volume and VolumeRatio per grain (either may be NULL). */
+mpf_status mpf_get_grains(mpf_ctx* ctx, double* volume, double* volume_ratio);
+
+/* InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556) evaluated from
+ * the phase-pair maxima of energy/mobility (ISO models). */
+mpf_status mpf_max_time_step(mpf_ctx* ctx, double* dt_max);
+
+/* Number of limiter plausibility warnings (PhaseField.cpp:1536-1561) since creation. */
+mpf_status mpf_limiter_warnings(mpf_ctx* ctx, int64_t* count);
+
+/* First cell (linear index) that overflowed the slot capacity, -1 if none. */
+mpf_status mpf_overflow_cell(mpf_ctx* ctx, int64_t* cell);
+
+/* Device pointers of the current state (for device-resident consumers): counts (uint8),
+ * indices (uint16, slot-major [slot*ncells + cell]), values (double or float by precision),
+ * and the HIP stream (hipStream_t) the context runs on. Any output pointer may be NULL. */
+mpf_status mpf_device_state(mpf_ctx* ctx, void** count, void** index, void** value, void** stream);
+
+/* Kernel timing of the last mpf_step call: average duration in ms of the fused step kernel and
+ * of the grain-volume kernel, measured with HIP events on the context's stream. */
+mpf_status mpf_last_timing(mpf_ctx* ctx, double* step_kernel_ms, double* grain_kernel_ms);
+
+/* Enables/disables per-kernel HIP event timing in mpf_step (off by default). */
+mpf_status mpf_enable_timing(mpf_ctx* ctx, int32_t on);
+
+/* Human-readable description of the last error on this context (static storage per context). */
+const char* mpf_error_message(mpf_ctx* ctx);
+
+void mpf_destroy(mpf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,637 @@
+// Host side of the C ABI declared in include/mpf.h.
+//
+// Owns the device state (two slot-major SoA buffers, swapped every step), the grain table and the
+// phase-pair tables, and enqueues the kernels of mpf_kernels.hip on the context's stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mpf.h"
+#include "mpf_device.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+template <typename T> hipError_t dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+template <typename T> void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Host-to-device copy of n elements; the first failure of a sequence is kept in `e`.
+template <typename T> void h2d(hipError_t& e, T* dst, const T* src, size_t n)
+{
+    if (e == hipSuccess) e = hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+struct mpf_ctx {
+    mpf_grid grid{};
+    int nx = 1, ny = 1, nz = 1, dnx = 0, dny = 0, dnz = 0, active = 0;
+    int64_t ncells = 0;
+    int slots = 8;
+    bool fp32 = false, grad = false, dg = false;
+    int nphases = 1;
+    std::vector<double> sigma, mu, eps, maxe, dgval, dgallowed;
+    std::vector<int32_t> emodel, dgmode;
+    double tj_factor = 0.0, tj_energy = 0.0;
+    int tj_model = 0;
+    int consider_nucleus = 1;
+    double nucleus_volume_factor = 1.0, regularization = 1.0;
+    double wl[4] = {0, 0, 0, 0}, wg[4] = {0, 0, 0, 0};
+    double prefactor = 0.0, dg_prefactor = 0.0;
+
+    hipStream_t stream = nullptr;
+    uint8_t* cnt[2] = {nullptr, nullptr};
+    uint16_t* id[2] = {nullptr, nullptr};
+    void* val[2] = {nullptr, nullptr};
+    int cur = 0;
+
+    int ngrains = 0;
+    int32_t* g_phase = nullptr;
+    double* g_vr = nullptr;
+    double* g_rot = nullptr;
+    uint8_t* g_solid = nullptr;
+    double* g_volume = nullptr;
+    double* g_maxvolume = nullptr;
+    double* g_refvolume = nullptr;
+    uint8_t* g_exist = nullptr;
+    int32_t* g_stage = nullptr;
+    unsigned long long* g_acc = nullptr;
+
+    double* d_sigma = nullptr;
+    double* d_mu = nullptr;
+    double* d_eps = nullptr;
+    double* d_maxe = nullptr;
+    int32_t* d_emodel = nullptr;
+    int32_t* d_dgmode = nullptr;
+    double* d_dgval = nullptr;
+    double* d_dgallowed = nullptr;
+
+    int32_t* d_status = nullptr;
+    unsigned long long* d_first_overflow = nullptr;
+    unsigned long long* d_warnings = nullptr;
+    uint32_t* d_work_list = nullptr;
+    unsigned* d_work_count = nullptr;
+
+    bool have_state = false, have_grains = false, initialized = false;
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    double last_step_ms = 0.0, last_grain_ms = 0.0;
+    std::string err;
+
+    mpf_status fail(mpf_status s, const std::string& msg)
+    {
+        err = msg;
+        return s;
+    }
+    mpf_status hip(hipError_t e, const char* what)
+    {
+        if (e == hipSuccess) return MPF_OK;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return MPF_ERR_HIP;
+    }
+    size_t vsize() const { return fp32 ? sizeof(float) : sizeof(double); }
+};
+
+namespace {
+
+void free_grains(mpf_ctx* c)
+{
+    dfree(c->g_phase); dfree(c->g_vr); dfree(c->g_rot); dfree(c->g_solid);
+    dfree(c->g_volume); dfree(c->g_maxvolume); dfree(c->g_refvolume);
+    dfree(c->g_exist); dfree(c->g_stage); dfree(c->g_acc);
+}
+
+mpf::StepArgs step_args(const mpf_ctx* c, double dt, int src)
+{
+    mpf::StepArgs a{};
+    a.ncells = c->ncells;
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
+    a.dnx = c->dnx; a.dny = c->dny; a.dnz = c->dnz;
+    a.slots = c->slots;
+    for (int i = 0; i < 4; ++i) { a.wl[i] = c->wl[i]; a.wg[i] = c->wg[i]; }
+    a.prefactor = c->prefactor;
+    a.dg_prefactor = c->dg_prefactor;
+    a.dt = dt;
+    a.nphases = c->nphases;
+    a.consider_nucleus = c->consider_nucleus;
+    a.tj_model = c->tj_model;
+    a.tj_factor = c->tj_factor;
+    a.tj_energy = c->tj_energy;
+    a.sigma = c->d_sigma; a.mu = c->d_mu; a.eps = c->d_eps; a.max_energy = c->d_maxe;
+    a.emodel = c->d_emodel; a.dg_mode = c->d_dgmode; a.dg_value = c->d_dgval; a.dg_allowed = c->d_dgallowed;
+    a.gphase = c->g_phase; a.gvr = c->g_vr; a.grot = c->g_rot; a.gsolid = c->g_solid;
+    a.cnt_in = c->cnt[src]; a.id_in = c->id[src]; a.val_in = c->val[src];
+    a.cnt_out = c->cnt[1 - src]; a.id_out = c->id[1 - src]; a.val_out = c->val[1 - src];
+    a.vol_acc = c->g_acc;
+    a.status = c->d_status;
+    a.first_overflow = c->d_first_overflow;
+    a.warnings = c->d_warnings;
+    a.work_list = c->d_work_list;
+    a.work_count = c->d_work_count;
+    return a;
+}
+
+mpf::GrainArgs grain_args(const mpf_ctx* c)
+{
+    mpf::GrainArgs g{};
+    g.ngrains = c->ngrains;
+    g.vol_acc = c->g_acc;
+    g.volume = c->g_volume;
+    g.maxvolume = c->g_maxvolume;
+    g.refvolume = c->g_refvolume;
+    g.vr = c->g_vr;
+    g.exist = c->g_exist;
+    g.stage = c->g_stage;
+    g.work_count = c->d_work_count;
+    return g;
+}
+
+mpf_status read_status(mpf_ctx* c)
+{
+    int32_t st[mpf::kStatusWords];
+    mpf_status s = c->hip(hipMemcpyAsync(st, c->d_status, sizeof(st), hipMemcpyDeviceToHost, c->stream),
+                          "status readback");
+    if (s) return s;
+    s = c->hip(hipStreamSynchronize(c->stream), "stream synchronize");
+    if (s) return s;
+    if (st[mpf::kStatusOverflow]) return c->fail(MPF_SLOT_OVERFLOW, "a cell needed more phase-field slots than the capacity");
+    if (st[mpf::kStatusEmpty]) return c->fail(MPF_EMPTY_NODE, "finalize erased every entry of a cell");
+    if (st[mpf::kStatusVolumeRange]) return c->fail(MPF_ERR_STATE, "phase-field value outside [0, 2) in the volume sum");
+    return MPF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** out)
+{
+    if (!grid || !params || !out) return MPF_ERR_ARG;
+    *out = nullptr;
+    mpf_ctx* c = new (std::nothrow) mpf_ctx();
+    if (!c) return MPF_ERR_NOMEM;
+    c->grid = *grid;
+    auto bail = [&](mpf_status s, const char* msg) {
+        std::fprintf(stderr, "mpf_create: %s\n", msg);
+        mpf_destroy(c);
+        return s;
+    };
+    if (grid->nx < 0 || grid->ny < 0 || grid->nz < 0 || grid->nx + grid->ny + grid->nz == 0)
+        return bail(MPF_ERR_ARG, "grid sizes must be >= 0 and not all zero");
+    if (!(grid->dx > 0.0) || !(grid->iwidth > 0.0)) return bail(MPF_ERR_ARG, "dx and iwidth must be positive");
+    if (grid->slots < 2 || grid->slots > mpf::kMaxSlots) return bail(MPF_ERR_ARG, "slots must be in [2, 16]");
+    if (grid->precision != MPF_FP64 && grid->precision != MPF_FP32) return bail(MPF_ERR_ARG, "unknown precision");
+    for (int i = 0; i < 6; ++i)
+        if (grid->bc[i] != MPF_BC_PERIODIC) return bail(MPF_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");
+    if (params->nphases < 1 || !params->sigma || !params->mu) return bail(MPF_ERR_ARG, "nphases, sigma and mu are required");
+    if (params->tj_model < 0 || params->tj_model > 2) return bail(MPF_ERR_ARG, "unknown triple-junction model");
+
+    c->dnx = grid->nx != 0; c->dny = grid->ny != 0; c->dnz = grid->nz != 0;
+    c->nx = grid->nx ? grid->nx : 1;
+    c->ny = grid->ny ? grid->ny : 1;
+    c->nz = grid->nz ? grid->nz : 1;
+    c->active = c->dnx + c->dny + c->dnz;
+    c->ncells = static_cast<int64_t>(c->nx) * c->ny * c->nz;
+    c->slots = grid->slots;
+    c->fp32 = grid->precision == MPF_FP32;
+    c->nphases = params->nphases;
+    const size_t np2 = static_cast<size_t>(c->nphases) * c->nphases;
+    c->sigma.assign(params->sigma, params->sigma + np2);
+    c->mu.assign(params->mu, params->mu + np2);
+    c->eps.assign(np2, 0.0);
+    if (params->epsilon) c->eps.assign(params->epsilon, params->epsilon + np2);
+    c->emodel.assign(np2, MPF_ENERGY_ISO);
+    if (params->energy_model) c->emodel.assign(params->energy_model, params->energy_model + np2);
+    c->grad = false;
+    for (size_t i = 0; i < np2; ++i) {
+        if (c->emodel[i] != MPF_ENERGY_ISO && c->emodel[i] != MPF_ENERGY_CUBIC)
+            return bail(MPF_ERR_UNSUPPORTED, "energy model must be ISO or CUBIC");
+        if (c->emodel[i] == MPF_ENERGY_CUBIC) c->grad = true;
+    }
+    // InterfaceEnergyModel::ReadInputIso/ReadInputCubic MaxEnergy (InterfaceEnergyModel.h:78-94)
+    c->maxe.resize(np2);
+    for (size_t i = 0; i < np2; ++i)
+        c->maxe[i] = c->emodel[i] == MPF_ENERGY_CUBIC ? c->sigma[i] * (1.0 + c->eps[i] * 1.41) : c->sigma[i];
+    c->tj_model = params->tj_model;
+    c->tj_factor = params->tj_factor;
+    c->tj_energy = params->tj_energy;
+    c->consider_nucleus = params->consider_nucleus_volume;
+    c->nucleus_volume_factor = params->nucleus_volume_factor;
+    c->regularization = params->regularization_factor;
+
+    const double eta = grid->iwidth * grid->dx;
+    c->prefactor = kPi * kPi / (eta * eta);
+    c->dg_prefactor = kPi / eta;
+
+    c->dg = params->dg_mode != nullptr && params->dg_value != nullptr;
+    c->dgmode.assign(np2, 0);
+    c->dgval.assign(np2, 0.0);
+    c->dgallowed.assign(np2, 1.0);
+    if (c->dg) {
+        c->dgmode.assign(params->dg_mode, params->dg_mode + np2);
+        c->dgval.assign(params->dg_value, params->dg_value + np2);
+        for (int a = 0; a < c->nphases; ++a)
+            for (int b = 0; b < c->nphases; ++b) {
+                const size_t k = static_cast<size_t>(a) * c->nphases + b;
+                if (a == b) c->dgmode[k] = 0;  // UserDrivingForce::ReadInput: same-phase pairs are NONE
+                // DrivingForce::MergePhaseFieldIncrementsSR limiter amplitude (src/DrivingForce.cpp:548-558)
+                double allowed = params->dg_limit * c->dg_prefactor * c->maxe[k];
+                allowed *= c->regularization;
+                c->dgallowed[k] = allowed;
+            }
+    }
+
+    // Stencil weights by number of non-zero offset components: Laplacian / dx^2, gradient / dx
+    // (LaplacianStencil.h:85-119, GradientStencil.h:58-92; 1D/2D use the middle slices).
+    const double dx = grid->dx;
+    double lt[4], gt[4];
+    if (c->active == 1) {
+        lt[0] = -2.0; lt[1] = 1.0; lt[2] = 0.0; lt[3] = 0.0;
+        gt[0] = 0.0; gt[1] = 0.5; gt[2] = 0.0; gt[3] = 0.0;
+    } else if (c->active == 2) {
+        lt[0] = -10.0 / 3.0; lt[1] = 2.0 / 3.0; lt[2] = 1.0 / 6.0; lt[3] = 0.0;
+        gt[0] = 0.0; gt[1] = 1.0 / 3.0; gt[2] = 0.25 / 3.0; gt[3] = 0.0;
+    } else {
+        lt[0] = -64.0 / 15.0; lt[1] = 7.0 / 15.0; lt[2] = 1.0 / 10.0; lt[3] = 1.0 / 30.0;
+        gt[0] = 0.0; gt[1] = 1.0 / 4.64; gt[2] = 0.245 / 4.64; gt[3] = 0.085 / 4.64;
+    }
+    for (int i = 0; i < 4; ++i) {
+        c->wl[i] = lt[i] / (dx * dx);
+        c->wg[i] = gt[i] / dx;
+    }
+
+    const size_t N = static_cast<size_t>(c->ncells);
+    mpf_status s = MPF_OK;
+    if ((s = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate"))) return bail(s, c->err.c_str());
+    for (int b = 0; b < 2; ++b) {
+        if ((s = c->hip(dalloc(&c->cnt[b], N), "hipMalloc counts"))) return bail(MPF_ERR_NOMEM, c->err.c_str());
+        if ((s = c->hip(dalloc(&c->id[b], N * c->slots), "hipMalloc indices"))) return bail(MPF_ERR_NOMEM, c->err.c_str());
+        if ((s = c->hip(hipMalloc(&c->val[b], N * c->slots * c->vsize()), "hipMalloc values"))) return bail(MPF_ERR_NOMEM, c->err.c_str());
+    }
+    if (c->hip(dalloc(&c->d_sigma, np2), "tables") || c->hip(dalloc(&c->d_mu, np2), "tables") ||
+        c->hip(dalloc(&c->d_eps, np2), "tables") || c->hip(dalloc(&c->d_maxe, np2), "tables") ||
+        c->hip(dalloc(&c->d_emodel, np2), "tables") || c->hip(dalloc(&c->d_dgmode, np2), "tables") ||
+        c->hip(dalloc(&c->d_dgval, np2), "tables") || c->hip(dalloc(&c->d_dgallowed, np2), "tables") ||
+        c->hip(dalloc(&c->d_status, mpf::kStatusWords), "status") ||
+        c->hip(dalloc(&c->d_first_overflow, 1), "status") || c->hip(dalloc(&c->d_warnings, 1), "status") ||
+        c->hip(dalloc(&c->d_work_list, N), "work list") || c->hip(dalloc(&c->d_work_count, 1), "work list"))
+        return bail(MPF_ERR_NOMEM, c->err.c_str());
+    hipError_t e = hipSuccess;
+    h2d(e, c->d_sigma, c->sigma.data(), np2);
+    h2d(e, c->d_mu, c->mu.data(), np2);
+    h2d(e, c->d_eps, c->eps.data(), np2);
+    h2d(e, c->d_maxe, c->maxe.data(), np2);
+    h2d(e, c->d_emodel, c->emodel.data(), np2);
+    h2d(e, c->d_dgmode, c->dgmode.data(), np2);
+    h2d(e, c->d_dgval, c->dgval.data(), np2);
+    h2d(e, c->d_dgallowed, c->dgallowed.data(), np2);
+    const int32_t zeros[mpf::kStatusWords] = {0, 0, 0, 0};
+    const unsigned long long none = ~0ull, zero = 0ull;
+    h2d(e, c->d_status, zeros, mpf::kStatusWords);
+    h2d(e, c->d_first_overflow, &none, 1);
+    h2d(e, c->d_warnings, &zero, 1);
+    const unsigned zero_u = 0u;
+    h2d(e, c->d_work_count, &zero_u, 1);
+    if ((s = c->hip(e, "init copies"))) return bail(s, c->err.c_str());
+    *out = c;
+    return MPF_OK;
+}
+
+mpf_status mpf_set_grains(mpf_ctx* c, int32_t ngrains, const int32_t* phase, const double* rotation,
+                          const uint8_t* solid)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (ngrains < 1 || ngrains > 65536) return c->fail(MPF_ERR_ARG, "ngrains must be in [1, 65536]");
+    free_grains(c);
+    c->ngrains = ngrains;
+    const size_t G = static_cast<size_t>(ngrains);
+    std::vector<int32_t> ph(G, 0);
+    std::vector<double> rot(9 * G, 0.0), ref(G), zero(G, 0.0);
+    std::vector<uint8_t> sol(G, 1), ex(G, 1);
+    std::vector<int32_t> stage(G, 2);
+    for (size_t g = 0; g < G; ++g) {
+        if (phase) {
+            if (phase[g] < 0 || phase[g] >= c->nphases) return c->fail(MPF_ERR_ARG, "grain phase out of range");
+            ph[g] = phase[g];
+        }
+        for (int q = 0; q < 9; ++q) rot[9 * g + q] = rotation ? rotation[9 * g + q] : (q % 4 == 0 ? 1.0 : 0.0);
+        if (solid) sol[g] = solid[g] ? 1 : 0;
+    }
+    // PhaseField::CalculateReferenceVolume(iWidth) (src/PhaseField.cpp:367-391)
+    const double r = c->grid.iwidth;
+    double rv = 0.0;
+    if (c->active == 1) rv = r;
+    else if (c->active == 2) rv = kPi * r * r;
+    else rv = (4.0 / 3.0) * kPi * r * r * r;
+    rv = rv * c->nucleus_volume_factor;
+    std::fill(ref.begin(), ref.end(), rv);
+
+    if (c->hip(dalloc(&c->g_phase, G), "grains") || c->hip(dalloc(&c->g_vr, G), "grains") ||
+        c->hip(dalloc(&c->g_rot, 9 * G), "grains") || c->hip(dalloc(&c->g_solid, G), "grains") ||
+        c->hip(dalloc(&c->g_volume, G), "grains") || c->hip(dalloc(&c->g_maxvolume, G), "grains") ||
+        c->hip(dalloc(&c->g_refvolume, G), "grains") || c->hip(dalloc(&c->g_exist, G), "grains") ||
+        c->hip(dalloc(&c->g_stage, G), "grains") || c->hip(dalloc(&c->g_acc, G * mpf::kLimbs), "grains"))
+        return MPF_ERR_NOMEM;
+    std::vector<unsigned long long> acc(G * mpf::kLimbs, 0ull);
+    hipError_t e = hipSuccess;
+    h2d(e, c->g_phase, ph.data(), G);
+    h2d(e, c->g_vr, zero.data(), G);
+    h2d(e, c->g_rot, rot.data(), 9 * G);
+    h2d(e, c->g_solid, sol.data(), G);
+    h2d(e, c->g_volume, zero.data(), G);
+    h2d(e, c->g_maxvolume, zero.data(), G);
+    h2d(e, c->g_refvolume, ref.data(), G);
+    h2d(e, c->g_exist, ex.data(), G);
+    h2d(e, c->g_stage, stage.data(), G);
+    h2d(e, c->g_acc, acc.data(), G * mpf::kLimbs);
+    mpf_status s = c->hip(e, "grain upload");
+    if (s) return s;
+    c->have_grains = true;
+    c->initialized = false;
+    return MPF_OK;
+}
+
+mpf_status mpf_upload(mpf_ctx* c, const int32_t* count, const int32_t* index, const double* value)
+{
+    if (!c || !count || !index || !value) return c ? c->fail(MPF_ERR_ARG, "null buffer") : MPF_ERR_ARG;
+    if (!c->have_grains) return c->fail(MPF_ERR_STATE, "mpf_set_grains must precede mpf_upload");
+    const size_t N = static_cast<size_t>(c->ncells);
+    const int K = c->slots;
+    std::vector<uint8_t> cnt(N);
+    std::vector<uint16_t> ids(N * K, 0);
+    std::vector<double> vd;
+    std::vector<float> vf;
+    if (c->fp32) vf.assign(N * K, 0.0f);
+    else vd.assign(N * K, 0.0);
+    size_t off = 0;
+    for (size_t i = 0; i < N; ++i) {
+        if (count[i] < 1 || count[i] > K)
+            return c->fail(count[i] < 1 ? MPF_ERR_ARG : MPF_SLOT_OVERFLOW, "cell " + std::to_string(i) + " has " +
+                           std::to_string(count[i]) + " entries (allowed 1.." + std::to_string(K) + ")");
+        cnt[i] = static_cast<uint8_t>(count[i]);
+        for (int s = 0; s < count[i]; ++s, ++off) {
+            if (index[off] < 0 || index[off] >= c->ngrains)
+                return c->fail(MPF_ERR_ARG, "phase-field index out of range in cell " + std::to_string(i));
+            ids[static_cast<size_t>(s) * N + i] = static_cast<uint16_t>(index[off]);
+            if (c->fp32) vf[static_cast<size_t>(s) * N + i] = static_cast<float>(value[off]);
+            else vd[static_cast<size_t>(s) * N + i] = value[off];
+        }
+    }
+    const int b = c->cur;
+    mpf_status s = c->hip(hipMemcpy(c->cnt[b], cnt.data(), N, hipMemcpyHostToDevice), "upload counts");
+    if (!s) s = c->hip(hipMemcpy(c->id[b], ids.data(), N * K * sizeof(uint16_t), hipMemcpyHostToDevice), "upload indices");
+    if (!s) s = c->hip(hipMemcpy(c->val[b], c->fp32 ? static_cast<const void*>(vf.data()) : static_cast<const void*>(vd.data()),
+                                 N * K * c->vsize(), hipMemcpyHostToDevice), "upload values");
+    if (s) return s;
+    c->have_state = true;
+    c->initialized = false;
+    return MPF_OK;
+}
+
+mpf_status mpf_finalize_initialization(mpf_ctx* c)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (!c->have_state || !c->have_grains) return c->fail(MPF_ERR_STATE, "upload state and grains first");
+    mpf::StepArgs a = step_args(c, 0.0, c->cur);
+    mpf_status s = c->hip(static_cast<hipError_t>(mpf::launch_finalize_init(a, c->fp32, c->stream)), "finalize kernel");
+    if (s) return s;
+    s = c->hip(static_cast<hipError_t>(mpf::launch_grains(grain_args(c), c->stream)), "grain kernel");
+    if (s) return s;
+    c->cur = 1 - c->cur;
+    s = read_status(c);
+    if (s) return s;
+    c->initialized = true;
+    return MPF_OK;
+}
+
+mpf_status mpf_step_async(mpf_ctx* c, double dt, int32_t nsteps)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization must precede mpf_step");
+    if (nsteps < 0 || !(dt > 0.0)) return c->fail(MPF_ERR_ARG, "dt must be > 0 and nsteps >= 0");
+    // Per-step events (timing on): before the register kernel, after it, after the deferred-cell
+    // and grain kernels.
+    const bool timing = c->timing && nsteps > 0;
+    if (timing && c->events.size() < static_cast<size_t>(3 * nsteps)) {
+        for (auto e : c->events) (void)hipEventDestroy(e);
+        c->events.assign(3 * static_cast<size_t>(nsteps), nullptr);
+        for (auto& e : c->events)
+            if (mpf_status st = c->hip(hipEventCreate(&e), "hipEventCreate")) return st;
+    }
+    auto record = [&](int i) -> mpf_status {
+        return timing ? c->hip(hipEventRecord(c->events[i], c->stream), "hipEventRecord") : MPF_OK;
+    };
+    const mpf::GrainArgs g = grain_args(c);
+    for (int s = 0; s < nsteps; ++s) {
+        mpf::StepArgs a = step_args(c, dt, c->cur);
+        mpf_status st = record(3 * s);
+        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_step(a, c->fp32, c->grad || c->dg, c->stream)), "step kernel");
+        if (!st) st = record(3 * s + 1);
+        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_wide(a, c->fp32, c->grad || c->dg, c->stream)), "deferred-cell kernel");
+        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_grains(g, c->stream)), "grain kernel");
+        if (!st) st = record(3 * s + 2);
+        if (st) return st;
+        c->cur = 1 - c->cur;
+    }
+    if (timing) {
+        if (mpf_status st = c->hip(hipEventSynchronize(c->events[3 * (nsteps - 1) + 2]), "hipEventSynchronize")) return st;
+        double ks = 0.0, kg = 0.0;
+        for (int s = 0; s < nsteps; ++s) {
+            float a = 0.f, b = 0.f;
+            if (mpf_status st = c->hip(hipEventElapsedTime(&a, c->events[3 * s], c->events[3 * s + 1]), "hipEventElapsedTime")) return st;
+            if (mpf_status st = c->hip(hipEventElapsedTime(&b, c->events[3 * s + 1], c->events[3 * s + 2]), "hipEventElapsedTime")) return st;
+            ks += a; kg += b;
+        }
+        c->last_step_ms = ks / nsteps;
+        c->last_grain_ms = kg / nsteps;
+    }
+    return MPF_OK;
+}
+
+mpf_status mpf_step(mpf_ctx* c, double dt, int32_t nsteps)
+{
+    mpf_status s = mpf_step_async(c, dt, nsteps);
+    if (s) return s;
+    return read_status(c);
+}
+
+mpf_status mpf_synchronize(mpf_ctx* c)
+{
+    if (!c) return MPF_ERR_ARG;
+    return read_status(c);
+}
+
+mpf_status mpf_total_entries(mpf_ctx* c, int64_t* total)
+{
+    if (!c || !total) return MPF_ERR_ARG;
+    if (!c->have_state) return c->fail(MPF_ERR_STATE, "no state");
+    std::vector<uint8_t> cnt(static_cast<size_t>(c->ncells));
+    mpf_status s = c->hip(hipMemcpyAsync(cnt.data(), c->cnt[c->cur], cnt.size(), hipMemcpyDeviceToHost, c->stream), "download counts");
+    if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s) return s;
+    int64_t t = 0;
+    for (uint8_t v : cnt) t += v;
+    *total = t;
+    return MPF_OK;
+}
+
+mpf_status mpf_download(mpf_ctx* c, int32_t* count, int32_t* index, double* value, int8_t* flag)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (!c->have_state) return c->fail(MPF_ERR_STATE, "no state");
+    const size_t N = static_cast<size_t>(c->ncells);
+    const int K = c->slots;
+    std::vector<uint8_t> cnt(N);
+    std::vector<uint16_t> ids(N * K);
+    std::vector<double> vd;
+    std::vector<float> vf;
+    if (c->fp32) vf.resize(N * K); else vd.resize(N * K);
+    const int b = c->cur;
+    mpf_status s = c->hip(hipMemcpyAsync(cnt.data(), c->cnt[b], N, hipMemcpyDeviceToHost, c->stream), "download");
+    if (!s) s = c->hip(hipMemcpyAsync(ids.data(), c->id[b], N * K * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream), "download");
+    if (!s) s = c->hip(hipMemcpyAsync(c->fp32 ? static_cast<void*>(vf.data()) : static_cast<void*>(vd.data()), c->val[b],
+                                      N * K * c->vsize(), hipMemcpyDeviceToHost, c->stream), "download");
+    if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s) return s;
+    size_t off = 0;
+    for (size_t i = 0; i < N; ++i) {
+        if (count) count[i] = cnt[i];
+        for (int q = 0; q < cnt[i]; ++q, ++off) {
+            if (index) index[off] = ids[static_cast<size_t>(q) * N + i];
+            if (value) value[off] = c->fp32 ? static_cast<double>(vf[static_cast<size_t>(q) * N + i]) : vd[static_cast<size_t>(q) * N + i];
+        }
+    }
+    if (flag) {
+        for (size_t i = 0; i < N; ++i) {
+            if (cnt[i] > 1) { flag[i] = 2; continue; }
+            const int x = static_cast<int>(i / (static_cast<size_t>(c->ny) * c->nz));
+            const int y = static_cast<int>((i / c->nz) % c->ny);
+            const int z = static_cast<int>(i % c->nz);
+            int8_t f = 0;
+            for (int ii = -c->dnx; ii <= c->dnx && !f; ++ii)
+                for (int jj = -c->dny; jj <= c->dny && !f; ++jj)
+                    for (int kk = -c->dnz; kk <= c->dnz && !f; ++kk) {
+                        const int xx = (x + ii + c->nx) % c->nx, yy = (y + jj + c->ny) % c->ny, zz = (z + kk + c->nz) % c->nz;
+                        const size_t nb = (static_cast<size_t>(xx) * c->ny + yy) * c->nz + zz;
+                        if (cnt[nb] > 1) f = 1;
+                    }
+            flag[i] = f;
+        }
+    }
+    return MPF_OK;
+}
+
+mpf_status mpf_get_grains(mpf_ctx* c, double* volume, double* volume_ratio)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (!c->have_grains) return c->fail(MPF_ERR_STATE, "no grains");
+    const size_t G = static_cast<size_t>(c->ngrains);
+    mpf_status s = MPF_OK;
+    if (volume) s = c->hip(hipMemcpyAsync(volume, c->g_volume, G * sizeof(double), hipMemcpyDeviceToHost, c->stream), "grains");
+    if (!s && volume_ratio) s = c->hip(hipMemcpyAsync(volume_ratio, c->g_vr, G * sizeof(double), hipMemcpyDeviceToHost, c->stream), "grains");
+    if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    return s;
+}
+
+mpf_status mpf_max_time_step(mpf_ctx* c, double* dt_max)
+{
+    if (!c || !dt_max) return MPF_ERR_ARG;
+    double me = 0.0, mm = 0.0;
+    for (size_t i = 0; i < c->maxe.size(); ++i) {
+        me = std::max(me, c->maxe[i]);
+        mm = std::max(mm, c->mu[i]);
+    }
+    double dt = DBL_MAX;
+    if (me > DBL_EPSILON && mm > 0.0) {
+        const double f = c->active == 1 ? 0.50 : (c->active == 2 ? 0.25 : 0.16);
+        dt = f * c->grid.dx * c->grid.dx / (mm * me * c->regularization);
+    }
+    *dt_max = dt;
+    return MPF_OK;
+}
+
+mpf_status mpf_limiter_warnings(mpf_ctx* c, int64_t* count)
+{
+    if (!c || !count) return MPF_ERR_ARG;
+    unsigned long long w = 0;
+    mpf_status s = c->hip(hipMemcpyAsync(&w, c->d_warnings, sizeof(w), hipMemcpyDeviceToHost, c->stream), "warnings");
+    if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    *count = static_cast<int64_t>(w);
+    return s;
+}
+
+mpf_status mpf_overflow_cell(mpf_ctx* c, int64_t* cell)
+{
+    if (!c || !cell) return MPF_ERR_ARG;
+    unsigned long long w = 0;
+    mpf_status s = c->hip(hipMemcpyAsync(&w, c->d_first_overflow, sizeof(w), hipMemcpyDeviceToHost, c->stream), "overflow");
+    if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    *cell = w == ~0ull ? -1 : static_cast<int64_t>(w);
+    return s;
+}
+
+mpf_status mpf_device_state(mpf_ctx* c, void** count, void** index, void** value, void** stream)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (count) *count = c->cnt[c->cur];
+    if (index) *index = c->id[c->cur];
+    if (value) *value = c->val[c->cur];
+    if (stream) *stream = c->stream;
+    return MPF_OK;
+}
+
+mpf_status mpf_last_timing(mpf_ctx* c, double* step_kernel_ms, double* grain_kernel_ms)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (step_kernel_ms) *step_kernel_ms = c->last_step_ms;
+    if (grain_kernel_ms) *grain_kernel_ms = c->last_grain_ms;
+    return MPF_OK;
+}
+
+mpf_status mpf_enable_timing(mpf_ctx* c, int32_t on)
+{
+    if (!c) return MPF_ERR_ARG;
+    c->timing = on != 0;
+    return MPF_OK;
+}
+
+const char* mpf_error_message(mpf_ctx* c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+void mpf_destroy(mpf_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->events) (void)hipEventDestroy(e);
+    for (int b = 0; b < 2; ++b) { dfree(c->cnt[b]); dfree(c->id[b]); dfree(c->val[b]); }
+    free_grains(c);
+    dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
+    dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
+    dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
+    dfree(c->d_work_list); dfree(c->d_work_count);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // extern "C"

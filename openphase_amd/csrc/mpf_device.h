@@ -1,0 +1,77 @@
+// Shared definitions of the MPF step kernels and the host library (no torch types).
+#pragma once
+
+#include <stdint.h>
+
+namespace mpf {
+
+constexpr int kMaxSlots = 16;                        // largest per-cell list capacity of the stored state
+constexpr int kBlock = 256;                          // threads per block of the step kernels
+constexpr int kWideBlock = 64;                       // threads per block of the deferred-cell kernel
+constexpr int kWideGrid = 512;                       // blocks of the deferred-cell kernel (grid-stride)
+constexpr int kLimbs = 4;                            // 32-bit limbs of the exact per-grain volume sums
+
+// Device status words (one int each).
+enum StatusWord { kStatusOverflow = 0, kStatusEmpty = 1, kStatusVolumeRange = 2, kStatusWords = 4 };
+
+struct StepArgs {
+    int64_t ncells;
+    int32_t nx, ny, nz;          // storage sizes (a reduced axis has size 1)
+    int32_t dnx, dny, dnz;       // 1 if the axis is active
+    int32_t slots;               // stored-list capacity (storage stride), <= kMaxSlots
+    double wl[4];                // Laplacian weight by number of non-zero offset components, / dx^2
+    double wg[4];                // gradient weight by number of non-zero offset components, / dx
+    double prefactor;            // Pi^2 / Eta^2
+    double dg_prefactor;         // Pi / Eta
+    double dt;
+    int32_t nphases;
+    int32_t consider_nucleus;
+    int32_t tj_model;            // 0 MODEL, 1 VALUE, 2 NONE
+    double tj_factor;
+    double tj_energy;
+    const double* sigma;         // nphases^2 phase-pair interface energy
+    const double* mu;            // nphases^2 phase-pair mobility
+    const double* eps;           // nphases^2 cubic anisotropy strength
+    const double* max_energy;    // nphases^2 MaxEnergy of the energy model
+    const int32_t* emodel;       // nphases^2 energy model (0 ISO, 1 CUBIC)
+    const int32_t* dg_mode;      // nphases^2, 1 = constant driving force for the pair
+    const double* dg_value;      // nphases^2 antisymmetric driving force
+    const double* dg_allowed;    // nphases^2 tanh limiter amplitude
+    const int32_t* gphase;       // per grain: thermodynamic phase
+    const double* gvr;           // per grain: VolumeRatio
+    const double* grot;          // per grain: rotation matrix, row-major
+    const uint8_t* gsolid;       // per grain: 1 if solid
+    const uint8_t* cnt_in;
+    const uint16_t* id_in;
+    const void* val_in;
+    uint8_t* cnt_out;
+    uint16_t* id_out;
+    void* val_out;
+    unsigned long long* vol_acc; // kLimbs per grain
+    int32_t* status;             // kStatusWords
+    unsigned long long* first_overflow;
+    unsigned long long* warnings;
+    uint32_t* work_list;         // cells deferred to the wide kernel (capacity ncells)
+    unsigned* work_count;
+};
+
+struct GrainArgs {
+    int32_t ngrains;
+    unsigned long long* vol_acc;
+    double* volume;
+    double* maxvolume;
+    const double* refvolume;
+    double* vr;
+    uint8_t* exist;
+    int32_t* stage;              // 0 seed, 1 nucleus, 2 stable
+    unsigned* work_count;        // reset here for the next step
+};
+
+// Host-side launchers (mpf_kernels.hip); return a hipError_t value.
+// `full`: orientation-dependent energies and/or constant driving force present.
+int launch_step(const StepArgs& a, bool fp32, bool full, void* stream);
+int launch_wide(const StepArgs& a, bool fp32, bool full, void* stream);
+int launch_finalize_init(const StepArgs& a, bool fp32, void* stream);
+int launch_grains(const GrainArgs& g, void* stream);
+
+}  // namespace mpf

@@ -1,0 +1,200 @@
+"""Engine: one device-resident multiphase-field state behind the C ABI (include/mpf.h).
+
+This is the numpy-level handle used by the OpenPhase-style classes (openphase_amd.openphase), the
+benchmark and the tests. All numerics run in libmpf_hip.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+from ._capi import MpfError, MpfGrid, MpfParams, ptr
+from .microstructure import Lists, storage_shape
+
+
+@dataclass
+class StepConfig:
+    """Grid, interface and driving-force parameters of one simulation (the .opi keys of
+    GridParameters, InterfaceProperties, PhaseField, UserDrivingForce and DrivingForce)."""
+
+    nx: int
+    ny: int
+    nz: int
+    dx: float = 1e-6
+    iwidth: float = 5.0
+    nphases: int = 1
+    sigma: np.ndarray = field(default_factory=lambda: np.array([[0.24]]))
+    mu: np.ndarray = field(default_factory=lambda: np.array([[4e-9]]))
+    energy_model: np.ndarray | None = None  # MPF_ENERGY_ISO / MPF_ENERGY_CUBIC per phase pair
+    epsilon: np.ndarray | None = None
+    tj_model: int = _capi.MPF_TJ_MODEL
+    tj_factor: float = 0.0
+    tj_energy: float = 0.0
+    consider_nucleus_volume: bool = True
+    nucleus_volume_factor: float = 1.0
+    dg_mode: np.ndarray | None = None  # 1 = VALUE per phase pair
+    dg_value: np.ndarray | None = None
+    dg_limit: float = 0.95
+    regularization_factor: float = 1.0
+    precision: int = _capi.MPF_FP64
+    slots: int = 8
+
+    @property
+    def shape(self) -> tuple:
+        return storage_shape(self.nx, self.ny, self.nz)
+
+    @property
+    def active(self) -> int:
+        return int(self.nx > 0) + int(self.ny > 0) + int(self.nz > 0)
+
+    def max_time_step(self) -> float:
+        """InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556) from the
+        phase-pair maxima of the tables."""
+        me = float(np.max(self._max_energy()))
+        mm = float(np.max(np.asarray(self.mu, dtype=np.float64)))
+        f = {1: 0.50, 2: 0.25, 3: 0.16}[self.active]
+        return f * self.dx * self.dx / (mm * me * self.regularization_factor)
+
+    def _max_energy(self) -> np.ndarray:
+        sig = np.asarray(self.sigma, dtype=np.float64)
+        if self.energy_model is None:
+            return sig
+        eps = np.zeros_like(sig) if self.epsilon is None else np.asarray(self.epsilon, dtype=np.float64)
+        return np.where(np.asarray(self.energy_model) == _capi.MPF_ENERGY_CUBIC, sig * (1.0 + eps * 1.41), sig)
+
+
+class Engine:
+    """Owns an mpf_ctx. Methods raise MpfError on any non-OK status."""
+
+    def __init__(self, cfg: StepConfig):
+        self.lib = _capi.load()
+        self.cfg = cfg
+        g = MpfGrid()
+        g.nx, g.ny, g.nz = cfg.nx, cfg.ny, cfg.nz
+        g.dx, g.iwidth = cfg.dx, cfg.iwidth
+        for i in range(6):
+            g.bc[i] = 0
+        g.precision = cfg.precision
+        g.slots = cfg.slots
+        np2 = cfg.nphases * cfg.nphases
+        self._keep = []  # arrays referenced by the params struct during mpf_create
+
+        def arr(a, dtype):
+            if a is None:
+                return None
+            x = np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(np2))
+            self._keep.append(x)
+            return x
+
+        p = MpfParams()
+        p.nphases = cfg.nphases
+        p.sigma = ptr(arr(cfg.sigma, np.float64), C.c_double)
+        p.mu = ptr(arr(cfg.mu, np.float64), C.c_double)
+        p.energy_model = ptr(arr(cfg.energy_model, np.int32), C.c_int32)
+        p.epsilon = ptr(arr(cfg.epsilon, np.float64), C.c_double)
+        p.tj_model, p.tj_factor, p.tj_energy = cfg.tj_model, cfg.tj_factor, cfg.tj_energy
+        p.consider_nucleus_volume = int(cfg.consider_nucleus_volume)
+        p.nucleus_volume_factor = cfg.nucleus_volume_factor
+        p.dg_mode = ptr(arr(cfg.dg_mode, np.int32), C.c_int32)
+        p.dg_value = ptr(arr(cfg.dg_value, np.float64), C.c_double)
+        p.dg_limit = cfg.dg_limit
+        p.regularization_factor = cfg.regularization_factor
+        h = C.c_void_p()
+        st = self.lib.mpf_create(C.byref(g), C.byref(p), C.byref(h))
+        if st != _capi.MPF_OK:
+            raise MpfError(st, "mpf_create")
+        self.h = h
+        self.ngrains = 0
+
+    # -- lifecycle --------------------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mpf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int, where: str) -> None:
+        if st != _capi.MPF_OK:
+            msg = self.lib.mpf_error_message(self.h)
+            raise MpfError(st, where, msg.decode() if msg else "")
+
+    # -- state ------------------------------------------------------------------------------
+    def set_grains(self, phase, rotation=None, solid=None) -> None:
+        phase = np.ascontiguousarray(phase, dtype=np.int32)
+        rot = None if rotation is None else np.ascontiguousarray(rotation, dtype=np.float64).reshape(-1)
+        sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8)
+        self._check(self.lib.mpf_set_grains(self.h, phase.shape[0], ptr(phase, C.c_int32), ptr(rot, C.c_double),
+                                            ptr(sol, C.c_uint8)), "mpf_set_grains")
+        self.ngrains = phase.shape[0]
+
+    def upload(self, lists: Lists) -> None:
+        if tuple(lists.shape) != self.cfg.shape:
+            raise ValueError(f"state shape {lists.shape} != grid {self.cfg.shape}")
+        cnt = np.ascontiguousarray(lists.count, dtype=np.int32)
+        idx = np.ascontiguousarray(lists.index, dtype=np.int32)
+        val = np.ascontiguousarray(lists.value, dtype=np.float64)
+        self._check(self.lib.mpf_upload(self.h, ptr(cnt, C.c_int32), ptr(idx, C.c_int32), ptr(val, C.c_double)),
+                    "mpf_upload")
+
+    def finalize_initialization(self) -> None:
+        self._check(self.lib.mpf_finalize_initialization(self.h), "mpf_finalize_initialization")
+
+    def step(self, dt: float, nsteps: int = 1) -> None:
+        self._check(self.lib.mpf_step(self.h, float(dt), int(nsteps)), "mpf_step")
+
+    def step_async(self, dt: float, nsteps: int = 1) -> None:
+        self._check(self.lib.mpf_step_async(self.h, float(dt), int(nsteps)), "mpf_step_async")
+
+    def synchronize(self) -> None:
+        self._check(self.lib.mpf_synchronize(self.h), "mpf_synchronize")
+
+    def download(self, with_flags: bool = False):
+        tot = C.c_int64()
+        self._check(self.lib.mpf_total_entries(self.h, C.byref(tot)), "mpf_total_entries")
+        n = int(np.prod(self.cfg.shape))
+        cnt = np.zeros(n, np.int32)
+        idx = np.zeros(tot.value, np.int32)
+        val = np.zeros(tot.value, np.float64)
+        flg = np.zeros(n, np.int8) if with_flags else None
+        self._check(self.lib.mpf_download(self.h, ptr(cnt, C.c_int32), ptr(idx, C.c_int32), ptr(val, C.c_double),
+                                          ptr(flg, C.c_int8)), "mpf_download")
+        lists = Lists(self.cfg.shape, cnt, idx, val)
+        return (lists, flg) if with_flags else lists
+
+    def grains(self):
+        vol = np.zeros(self.ngrains)
+        vr = np.zeros(self.ngrains)
+        self._check(self.lib.mpf_get_grains(self.h, ptr(vol, C.c_double), ptr(vr, C.c_double)), "mpf_get_grains")
+        return vol, vr
+
+    def limiter_warnings(self) -> int:
+        w = C.c_int64()
+        self._check(self.lib.mpf_limiter_warnings(self.h, C.byref(w)), "mpf_limiter_warnings")
+        return int(w.value)
+
+    def overflow_cell(self) -> int:
+        w = C.c_int64()
+        self._check(self.lib.mpf_overflow_cell(self.h, C.byref(w)), "mpf_overflow_cell")
+        return int(w.value)
+
+    def enable_timing(self, on: bool = True) -> None:
+        self._check(self.lib.mpf_enable_timing(self.h, int(on)), "mpf_enable_timing")
+
+    def last_timing(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.mpf_last_timing(self.h, C.byref(a), C.byref(b)), "mpf_last_timing")
+        return a.value, b.value
+
+    def device_state(self):
+        cnt, idx, val, stream = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self._check(self.lib.mpf_device_state(self.h, C.byref(cnt), C.byref(idx), C.byref(val), C.byref(stream)),
+                    "mpf_device_state")
+        return cnt.value, idx.value, val.value, stream.value

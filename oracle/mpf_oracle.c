@@ -1,0 +1,867 @@
+/*
+ * CPU oracle for the multiphase-field time step -- TEST INFRASTRUCTURE ONLY (see mpf_oracle.h).
+ *
+ * Every pass cites the reference loop it restates. The grid has no halo: the periodic halo copies
+ * of BoundaryConditions::SetX/Y/Z (include/BoundaryConditions.h:694-712, :825, :952) are replaced
+ * by wrapped neighbour indices, which is what those copies produce for N >= Bcells.
+ *
+ * Compile without -march and with -ffp-contract=off so that no FMA contraction happens (the
+ * reference's x86-64 build has no FMA either).
+ */
+#include "mpf_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.14159265358979323846
+
+typedef struct {
+    uint32_t idx;
+    double val;
+    double lap;
+    double grad[3];
+} orc_entry;
+
+typedef struct {
+    int32_t n;
+    int32_t flag;
+    orc_entry e[ORC_CAP];
+} orc_node;
+
+/* antisymmetric pair increment (NodeAB<double,double>, include/Containers/NodeAB.h:37-69) */
+typedef struct {
+    uint32_t a, b;
+    double v1, v2;
+} orc_pair;
+
+#define ORC_PCAP (ORC_CAP * ORC_CAP)
+
+typedef struct {
+    int di, dj, dk;
+    double w;          /* Laplacian weight / dx^2 */
+    double gx, gy, gz; /* gradient weights / dx */
+} orc_stencil;
+
+struct orc_ctx {
+    orc_params p;
+    int nx, ny, nz;       /* storage sizes (reduced axis -> 1) */
+    int dn[3];            /* active axes */
+    int active;
+    long ncells;
+    int nphases;
+    double* sigma;
+    double* mu;
+    int32_t* emodel;
+    double* eps;
+    double* dgval;
+    int32_t* dgmode;
+    int ngrains;
+    int32_t* gphase;
+    double* grot;
+    uint8_t* gsolid;
+    /* GrainsProperties (include/GrainsProperties.h:51-78) */
+    double* volume;
+    double* maxvolume;
+    double* refvolume;
+    double* vr;
+    uint8_t* exist;
+    int* stage; /* 0 seed, 1 nucleus, 2 stable (Grain::Clear sets Stable) */
+    orc_node* nodes;
+    orc_node* tmp;
+    int nls, ngs;
+    orc_stencil ls[27];
+    orc_stencil gs[27];
+    int need_grad;
+    int64_t warnings;
+    int max_list;   /* longest neighbour-merged list seen (entries incl. zero-valued) */
+    int max_final;  /* longest finalized list seen */
+    int status;
+    double max_energy, max_mobility; /* over phase pairs of the last IP.Set */
+};
+
+static long cell_index(const orc_ctx* c, long i, long j, long k)
+{
+    return (i * c->ny + j) * c->nz + k;
+}
+
+static long wrap(long i, long n)
+{
+    long r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+/* Stencil tables: weights depend only on how many offset components are non-zero.
+ * Laplacian: 1D three-point, 2D nine-point (Hale), 3D 27-point (Spotz-Carey);
+ * gradient: 1D central, 2D eight-point and 3D 26-point (Alfaraj, Wang, Luo 2014).
+ * Same numbers as include/Containers/LaplacianStencil.h:44-119 and GradientStencil.h:44-92;
+ * enumeration order x outer, z inner, zero weights skipped (LaplacianStencil.h:197-219,
+ * GradientStencil.h:132-156). */
+static double lap_table(int active, int nnz)
+{
+    if (active == 1) return nnz == 0 ? -2.0 : (nnz == 1 ? 1.0 : 0.0);
+    if (active == 2) return nnz == 0 ? -10.0 / 3.0 : (nnz == 1 ? 2.0 / 3.0 : 1.0 / 6.0);
+    switch (nnz) {
+    case 0: return -64.0 / 15.0;
+    case 1: return 7.0 / 15.0;
+    case 2: return 1.0 / 10.0;
+    default: return 1.0 / 30.0;
+    }
+}
+
+static double grad_table(int active, int nnz)
+{
+    if (active == 1) return nnz == 1 ? 0.5 : 0.0;
+    if (active == 2) return nnz == 0 ? 0.0 : (nnz == 1 ? 1.0 / 3.0 : 0.25 / 3.0);
+    switch (nnz) {
+    case 0: return 0.0;
+    case 1: return 1.0 / 4.64;
+    case 2: return 0.245 / 4.64;
+    default: return 0.085 / 4.64;
+    }
+}
+
+static void build_stencils(orc_ctx* c)
+{
+    double dx = c->p.dx;
+    c->nls = 0;
+    c->ngs = 0;
+    for (int x = -c->dn[0]; x <= c->dn[0]; ++x)
+    for (int y = -c->dn[1]; y <= c->dn[1]; ++y)
+    for (int z = -c->dn[2]; z <= c->dn[2]; ++z) {
+        int nnz = (x != 0) + (y != 0) + (z != 0);
+        double lw = lap_table(c->active, nnz);
+        if (lw != 0.0) {
+            orc_stencil* s = &c->ls[c->nls++];
+            s->di = x; s->dj = y; s->dk = z;
+            s->w = lw / (dx * dx);
+        }
+        double gw = grad_table(c->active, nnz);
+        if (gw != 0.0) {
+            orc_stencil* s = &c->gs[c->ngs++];
+            s->di = x; s->dj = y; s->dk = z;
+            s->gx = x * gw / dx;
+            s->gy = y * gw / dx;
+            s->gz = z * gw / dx;
+        }
+    }
+}
+
+/* PhaseField::CalculateReferenceVolume (src/PhaseField.cpp:367-391) with radius = iWidth */
+static double reference_volume(const orc_ctx* c)
+{
+    double r = c->p.iwidth, v = 0.0;
+    if (c->active == 1) v = r;
+    else if (c->active == 2) v = ORC_PI * r * r;
+    else v = (4.0 / 3.0) * ORC_PI * r * r * r;
+    return v * c->p.nucleus_volume_factor;
+}
+
+static double* dup_d(const double* src, size_t n, double fill)
+{
+    double* d = (double*)malloc(n * sizeof(double));
+    if (!d) return NULL;
+    for (size_t i = 0; i < n; ++i) d[i] = src ? src[i] : fill;
+    return d;
+}
+
+int orc_create(const orc_params* p, int32_t ngrains, const int32_t* grain_phase,
+               const double* grain_rot, const uint8_t* grain_solid, orc_ctx** out)
+{
+    if (!p || !out || ngrains < 0 || p->nphases < 1 || p->dx <= 0.0) return ORC_ERR_ARG;
+    if (p->nx < 0 || p->ny < 0 || p->nz < 0 || p->nx + p->ny + p->nz == 0) return ORC_ERR_ARG;
+    orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
+    if (!c) return ORC_ERR_NOMEM;
+    c->p = *p;
+    c->dn[0] = p->nx != 0; c->dn[1] = p->ny != 0; c->dn[2] = p->nz != 0;
+    c->nx = p->nx ? p->nx : 1;
+    c->ny = p->ny ? p->ny : 1;
+    c->nz = p->nz ? p->nz : 1;
+    c->active = c->dn[0] + c->dn[1] + c->dn[2];
+    c->ncells = (long)c->nx * c->ny * c->nz;
+    c->nphases = p->nphases;
+    size_t np2 = (size_t)p->nphases * p->nphases;
+    c->sigma = dup_d(p->sigma, np2, 0.0);
+    c->mu = dup_d(p->mu, np2, 0.0);
+    c->eps = dup_d(p->epsilon, np2, 0.0);
+    c->dgval = p->dg_value ? dup_d(p->dg_value, np2, 0.0) : NULL;
+    c->dgmode = (int32_t*)calloc(np2, sizeof(int32_t));
+    for (size_t i = 0; i < np2 && c->dgmode; ++i) c->dgmode[i] = (p->dg_mode && p->dg_value) ? p->dg_mode[i] : 0;
+    c->emodel = (int32_t*)calloc(np2, sizeof(int32_t));
+    c->need_grad = 0;
+    for (size_t i = 0; i < np2; ++i) {
+        c->emodel[i] = p->energy_model ? p->energy_model[i] : ORC_EN_ISO;
+        if (c->emodel[i] != ORC_EN_ISO) c->need_grad = 1;
+    }
+    c->ngrains = ngrains;
+    c->gphase = (int32_t*)calloc(ngrains ? ngrains : 1, sizeof(int32_t));
+    c->grot = (double*)calloc(9 * (size_t)(ngrains ? ngrains : 1), sizeof(double));
+    c->gsolid = (uint8_t*)calloc(ngrains ? ngrains : 1, 1);
+    c->volume = (double*)calloc(ngrains ? ngrains : 1, sizeof(double));
+    c->maxvolume = (double*)calloc(ngrains ? ngrains : 1, sizeof(double));
+    c->refvolume = (double*)calloc(ngrains ? ngrains : 1, sizeof(double));
+    c->vr = (double*)calloc(ngrains ? ngrains : 1, sizeof(double));
+    c->exist = (uint8_t*)calloc(ngrains ? ngrains : 1, 1);
+    c->stage = (int*)calloc(ngrains ? ngrains : 1, sizeof(int));
+    c->nodes = (orc_node*)calloc((size_t)c->ncells, sizeof(orc_node));
+    c->tmp = (orc_node*)calloc((size_t)c->ncells, sizeof(orc_node));
+    if (!c->sigma || !c->mu || !c->eps || !c->emodel || !c->gphase || !c->grot || !c->nodes || !c->tmp) {
+        orc_destroy(c);
+        return ORC_ERR_NOMEM;
+    }
+    double refv = reference_volume(c);
+    for (int g = 0; g < ngrains; ++g) {
+        c->gphase[g] = grain_phase ? grain_phase[g] : 0;
+        if (c->gphase[g] < 0 || c->gphase[g] >= p->nphases) { orc_destroy(c); return ORC_ERR_ARG; }
+        for (int q = 0; q < 9; ++q)
+            c->grot[9 * g + q] = grain_rot ? grain_rot[9 * g + q] : ((q % 4) == 0 ? 1.0 : 0.0);
+        c->gsolid[g] = grain_solid ? grain_solid[g] : 1;
+        /* PhaseField::AddGrainInfo (src/PhaseField.cpp:2970-2979) after GrainsProperties::add_grain */
+        c->refvolume[g] = refv;
+        c->vr[g] = 0.0;
+        c->exist[g] = 1;
+        c->stage[g] = 2;
+    }
+    build_stencils(c);
+    *out = c;
+    return ORC_OK;
+}
+
+void orc_destroy(orc_ctx* c)
+{
+    if (!c) return;
+    free(c->sigma); free(c->mu); free(c->eps); free(c->emodel); free(c->dgval); free(c->dgmode);
+    free(c->gphase); free(c->grot); free(c->gsolid);
+    free(c->volume); free(c->maxvolume); free(c->refvolume); free(c->vr);
+    free(c->exist); free(c->stage);
+    free(c->nodes); free(c->tmp);
+    free(c);
+}
+
+void orc_set_threads(int32_t n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_load(orc_ctx* c, const int32_t* count, const int32_t* index, const double* value)
+{
+    long off = 0;
+    for (long i = 0; i < c->ncells; ++i) {
+        orc_node* nd = &c->nodes[i];
+        if (count[i] < 0 || count[i] > ORC_CAP) return ORC_ERR_CAPACITY;
+        nd->n = count[i];
+        /* NodePF::read (include/Containers/NodePF.h:751-763): flag 2 when more than one entry */
+        nd->flag = count[i] > 1 ? 2 : 0;
+        for (int s = 0; s < count[i]; ++s, ++off) {
+            if (index[off] < 0 || index[off] >= c->ngrains) return ORC_ERR_ARG;
+            nd->e[s].idx = (uint32_t)index[off];
+            nd->e[s].val = value[off];
+            nd->e[s].lap = 0.0;
+            nd->e[s].grad[0] = nd->e[s].grad[1] = nd->e[s].grad[2] = 0.0;
+        }
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Pair list helpers (NodeAB semantics, include/Containers/NodeAB.h:321-547, 597-743)          */
+
+static double ab_get_asym1(const orc_pair* d, int nd, uint32_t n, uint32_t m)
+{
+    for (int i = 0; i < nd; ++i) {
+        if (d[i].a == n && d[i].b == m) return d[i].v1;
+        if (d[i].a == m && d[i].b == n) return d[i].v1 * (-1.0);
+    }
+    return 0.0;
+}
+
+static double ab_get_asym2(const orc_pair* d, int nd, uint32_t n, uint32_t m)
+{
+    for (int i = 0; i < nd; ++i) {
+        if (d[i].a == n && d[i].b == m) return d[i].v2;
+        if (d[i].a == m && d[i].b == n) return d[i].v2 * (-1.0);
+    }
+    return 0.0;
+}
+
+static int ab_set_sym_pair(orc_pair* d, int* nd, uint32_t n, uint32_t m, double v1, double v2)
+{
+    for (int i = 0; i < *nd; ++i)
+        if ((d[i].a == n && d[i].b == m) || (d[i].a == m && d[i].b == n)) {
+            d[i].v1 = v1;
+            d[i].v2 = v2;
+            return 0;
+        }
+    if (*nd >= ORC_PCAP) return -1;
+    d[*nd].a = n; d[*nd].b = m; d[*nd].v1 = v1; d[*nd].v2 = v2;
+    ++*nd;
+    return 0;
+}
+
+static int ab_add_asym1(orc_pair* d, int* nd, uint32_t n, uint32_t m, double v)
+{
+    for (int i = 0; i < *nd; ++i) {
+        if (d[i].a == n && d[i].b == m) { d[i].v1 += v; return 0; }
+        if (d[i].a == m && d[i].b == n) { d[i].v1 += v * (-1.0); return 0; }
+    }
+    if (*nd >= ORC_PCAP) return -1;
+    d[*nd].a = n; d[*nd].b = m; d[*nd].v1 = v; d[*nd].v2 = 0.0;
+    ++*nd;
+    return 0;
+}
+
+static double std_min(double a, double b) { return (b < a) ? b : a; }
+static double std_max(double a, double b) { return (a < b) ? b : a; }
+
+static double node_get_value(const orc_node* nd, uint32_t idx)
+{
+    for (int i = 0; i < nd->n; ++i) if (nd->e[i].idx == idx) return nd->e[i].val;
+    return 0.0;
+}
+
+/* NodePF::add_value (include/Containers/NodePF.h:288-306) */
+static int node_add_value(orc_node* nd, uint32_t idx, double v)
+{
+    for (int i = 0; i < nd->n; ++i)
+        if (nd->e[i].idx == idx) { nd->e[i].val += v; return 0; }
+    if (nd->n >= ORC_CAP) return -1;
+    if (nd->n) nd->flag = 2;
+    orc_entry* e = &nd->e[nd->n++];
+    e->idx = idx; e->val = v; e->lap = 0.0; e->grad[0] = e->grad[1] = e->grad[2] = 0.0;
+    return 0;
+}
+
+/* InterfaceOrientation (src/InterfaceProperties.cpp:39-45): R^T * normalize(-grad) */
+static void interface_orientation(const orc_ctx* c, const orc_entry* e, double n[3])
+{
+    double v[3] = {e->grad[0] * (-1.0), e->grad[1] * (-1.0), e->grad[2] * (-1.0)};
+    double len = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len > DBL_EPSILON) {
+        double inv = 1.0 / len;
+        v[0] *= inv; v[1] *= inv; v[2] *= inv;
+    } else {
+        v[0] = v[1] = v[2] = 0.0;
+    }
+    const double* R = &c->grot[9 * e->idx];
+    for (int i = 0; i < 3; ++i) n[i] = R[0 * 3 + i] * v[0] + R[1 * 3 + i] * v[1] + R[2 * 3 + i] * v[2];
+}
+
+/* InterfaceEnergyModel::Calculate, Cubic branch (include/InterfaceProperties/InterfaceEnergyModel.h:325-345) */
+static double cubic_energy(double energy, double eps1, const double n[3])
+{
+    return energy * (1.0 + eps1 * (1.5 - 2.5 * (pow(n[0], 4) + pow(n[1], 4) + pow(n[2], 4))));
+}
+
+static double max_energy_of(const orc_ctx* c, int pa, int pb)
+{
+    int k = pa * c->nphases + pb;
+    if (c->emodel[k] == ORC_EN_CUBIC) return c->sigma[k] * (1.0 + c->eps[k] * 1.41);
+    return c->sigma[k];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* One node: IP.Set -> DO increments (-> driving force) -> NormalizeIncrements -> MergeIncrements */
+
+typedef struct {
+    double max_energy, max_mobility;
+    int64_t warnings;
+    int err;
+} local_result;
+
+static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
+{
+    const int n = nd->n;
+    double sig[ORC_CAP][ORC_CAP];
+    double mob[ORC_CAP][ORC_CAP];
+    memset(sig, 0, sizeof(sig));
+    memset(mob, 0, sizeof(mob));
+
+    /* InterfaceProperties::SetSR (src/InterfaceProperties.cpp:603-762), ISO and CUBIC energy,
+     * ISO mobility; Properties(i,j,k) is cleared first so a missing pair reads 0. */
+    for (int a = 0; a < n; ++a)
+    for (int b = a + 1; b < n; ++b) {
+        const orc_entry* ea = &nd->e[a];
+        const orc_entry* eb = &nd->e[b];
+        int pa = c->gphase[ea->idx], pb = c->gphase[eb->idx];
+        int k = pa * c->nphases + pb;
+        double en = 0.0;
+        if (c->emodel[k] == ORC_EN_ISO) {
+            en = c->sigma[k];
+        } else {
+            double wa = 0.0, wb = 0.0;
+            if (c->gsolid[ea->idx]) {
+                wa = fabs(ea->val * (1.0 - ea->val));
+                if (wa > DBL_EPSILON) {
+                    double nrm[3];
+                    interface_orientation(c, ea, nrm);
+                    en += cubic_energy(c->sigma[k], c->eps[k], nrm) * wa;
+                } else {
+                    wa = 0.0;
+                }
+            }
+            if (c->gsolid[eb->idx]) {
+                wb = fabs(eb->val * (1.0 - eb->val));
+                if (wb > DBL_EPSILON) {
+                    double nrm[3];
+                    interface_orientation(c, eb, nrm);
+                    en += cubic_energy(c->sigma[k], c->eps[k], nrm) * wb;
+                } else {
+                    wb = 0.0;
+                }
+            }
+            if (wa + wb > DBL_EPSILON) en /= wa + wb;
+            else en = max_energy_of(c, pa, pb);
+        }
+        double mo = c->mu[k];
+        if (en != 0.0 || mo != 0.0) {
+            sig[a][b] = sig[b][a] = en;
+            mob[a][b] = mob[b][a] = mo;
+        }
+        res->max_energy = std_max(res->max_energy, en);
+        res->max_mobility = std_max(res->max_mobility, mo);
+    }
+
+    /* DoubleObstacle::CalculatePhaseFieldIncrementsSR (src/DoubleObstacle.cpp:502-571) */
+    orc_pair dot[ORC_PCAP];
+    int nd_dot = 0;
+    const double eta = c->p.iwidth * c->p.dx;
+    const double prefactor = ORC_PI * ORC_PI / (eta * eta);
+    double localn = 0.0;
+    if (c->p.consider_nucleus_volume) {
+        for (int a = 0; a < n; ++a) localn += c->vr[nd->e[a].idx];
+    } else {
+        localn = (double)n;
+    }
+    const double norm_1 = 1.0 / localn;
+    for (int a = 0; a < n; ++a)
+    for (int b = a + 1; b < n; ++b) {
+        const orc_entry* ea = &nd->e[a];
+        const orc_entry* eb = &nd->e[b];
+        double scale = sqrt(c->vr[ea->idx] * c->vr[eb->idx]);
+        double d = sig[a][b] * ((ea->lap + 0.5 * (1.0 + scale) * prefactor * ea->val) -
+                                (eb->lap + 0.5 * (1.0 + scale) * prefactor * eb->val));
+        if (n > 2)
+        for (int g = 0; g < n; ++g) {
+            if (g == a || g == b) continue;
+            const orc_entry* eg = &nd->e[g];
+            d += (sig[b][g] - sig[a][g]) * (eg->lap + 0.5 * (1.0 + scale) * prefactor * eg->val);
+            if (c->p.tj_model == ORC_TJ_MODEL) {
+                if (c->p.tj_factor != 0.0 || scale < 1.0) {
+                    int pa = c->gphase[ea->idx], pb = c->gphase[eb->idx], pg = c->gphase[eg->idx];
+                    int np = c->nphases;
+                    d += std_max(c->p.tj_factor, (1.0 - scale)) * prefactor *
+                         (c->sigma[pa * np + pb] + c->sigma[pb * np + pg] + c->sigma[pa * np + pg]) *
+                         (ea->val * eg->val - eb->val * eg->val);
+                }
+            } else if (c->p.tj_model == ORC_TJ_VALUE) {
+                d += c->p.tj_energy * prefactor * (ea->val * eg->val - eb->val * eg->val);
+            }
+        }
+        d *= mob[a][b] * norm_1 * scale;
+        if (ab_add_asym1(dot, &nd_dot, ea->idx, eb->idx, d)) { res->err = ORC_ERR_CAPACITY; return; }
+    }
+
+    /* UserDrivingForce::SetDrivingForce (src/UserDrivingForce.cpp:152-175) on interface nodes,
+     * DrivingForce::SkipAverage (src/DrivingForce.cpp:257-274) and
+     * DrivingForce::MergePhaseFieldIncrementsSR (src/DrivingForce.cpp:497-602), Limiting on. */
+    if (c->dgval && nd->flag == 2) {
+        const double pref = ORC_PI / eta;
+        for (int a = 0; a < n - 1; ++a)
+        for (int b = a + 1; b < n; ++b) {
+            int pa = c->gphase[nd->e[a].idx], pb = c->gphase[nd->e[b].idx];
+            if (c->dgmode[pa * c->nphases + pb] != 1) continue; /* no Force entry for this pair */
+            double raw = c->dgval[pa * c->nphases + pb];
+            double weight = sqrt(nd->e[a].val * nd->e[b].val);
+            double allowed = c->p.dg_limit * pref * max_energy_of(c, pa, pb);
+            allowed *= c->p.regularization_factor;
+            double dg = allowed * tanh(raw / allowed);
+            double dpsi = dg * mob[a][b] * weight * pref;
+            if (ab_add_asym1(dot, &nd_dot, nd->e[a].idx, nd->e[b].idx, dpsi)) { res->err = ORC_ERR_CAPACITY; return; }
+        }
+    }
+
+    /* PhaseField::NormalizeIncrementsSR (src/PhaseField.cpp:1351-1567) */
+    if (nd_dot == 1) {
+        uint32_t ia = dot[0].a;
+        double va = (dot[0].v1 + dot[0].v2) * dt;
+        double old = node_get_value(nd, ia);
+        if ((old == 0.0 && va < 0.0) || (old == 1.0 && va > 0.0)) {
+            nd_dot = 0;
+        } else {
+            double nv = old + va;
+            double norm = 1.0;
+            if (nv < 0.0) norm *= -old / va;
+            else if (nv > 1.0) norm *= (1.0 - old) / va;
+            if (norm > DBL_EPSILON) {
+                dot[0].v1 *= norm;
+                dot[0].v2 *= norm;
+            } else {
+                nd_dot = 0;
+            }
+        }
+    } else if (nd_dot > 1) {
+        for (int a = 0; a < n; ++a) {
+            double dpsi = 0.0;
+            for (int b = 0; b < n; ++b) {
+                if (a == b) continue;
+                dpsi += ab_get_asym1(dot, nd_dot, nd->e[a].idx, nd->e[b].idx);
+                dpsi += ab_get_asym2(dot, nd_dot, nd->e[a].idx, nd->e[b].idx);
+            }
+            if ((nd->e[a].val == 0.0 && dpsi < 0.0) || (nd->e[a].val == 1.0 && dpsi > 0.0)) {
+                for (int b = 0; b < n; ++b) {
+                    if (a == b) continue;
+                    if (ab_set_sym_pair(dot, &nd_dot, nd->e[a].idx, nd->e[b].idx, 0.0, 0.0)) {
+                        res->err = ORC_ERR_CAPACITY;
+                        return;
+                    }
+                }
+            }
+        }
+        int w = 0;
+        for (int i = 0; i < nd_dot; ++i)
+            if (!(dot[i].v1 == 0.0 && dot[i].v2 == 0.0)) dot[w++] = dot[i];
+        nd_dot = w;
+
+        if (nd_dot) {
+            /* per-index positive/negative sums and limits keyed by phase-field index */
+            double pos[ORC_CAP], neg[ORC_CAP], lim1[ORC_CAP], lim2[ORC_CAP];
+            int iter = 0;
+            int needed = 1;
+            while (needed) {
+                ++iter;
+                needed = 0;
+                for (int a = 0; a < n; ++a) { pos[a] = 0.0; neg[a] = 0.0; }
+                for (int i = 0; i < nd_dot; ++i) {
+                    int sa = -1, sb = -1;
+                    for (int a = 0; a < n; ++a) {
+                        if (nd->e[a].idx == dot[i].a) sa = a;
+                        if (nd->e[a].idx == dot[i].b) sb = a;
+                    }
+                    if (dot[i].v1 < 0.0) {
+                        neg[sa] += dot[i].v1;
+                        pos[sb] += -dot[i].v1;
+                    }
+                    if (dot[i].v1 > 0.0) {
+                        pos[sa] += dot[i].v1;
+                        neg[sb] += -dot[i].v1;
+                    }
+                }
+                for (int a = 0; a < n; ++a) {
+                    double phi = nd->e[a].val;
+                    double npf = phi + (pos[a] + neg[a]) * dt;
+                    lim2[a] = 1.0;
+                    lim1[a] = 1.0;
+                    if (npf < 0.0) lim2[a] = std_min(lim2[a], -(phi + pos[a] * dt) / (neg[a] * dt));
+                    if (npf > 1.0) lim1[a] = std_min(lim1[a], (1.0 - (phi + neg[a] * dt)) / (pos[a] * dt));
+                }
+                for (int i = 0; i < nd_dot; ++i) {
+                    int sa = -1, sb = -1;
+                    for (int a = 0; a < n; ++a) {
+                        if (nd->e[a].idx == dot[i].a) sa = a;
+                        if (nd->e[a].idx == dot[i].b) sb = a;
+                    }
+                    if (dot[i].v1 < 0.0) {
+                        double t = std_min(lim2[sa], lim1[sb]);
+                        dot[i].v1 *= t;
+                        if (t < 1.0) needed = 1;
+                    }
+                    if (dot[i].v1 > 0.0) {
+                        double t = std_min(lim1[sa], lim2[sb]);
+                        dot[i].v1 *= t;
+                        if (t < 1.0) needed = 1;
+                    }
+                }
+                if (iter > 24) needed = 0;
+            }
+            w = 0;
+            for (int i = 0; i < nd_dot; ++i)
+                if (!(fabs(dot[i].v1 * dt) < DBL_EPSILON)) dot[w++] = dot[i];
+            nd_dot = w;
+
+            /* plausibility check: counts the warnings the reference prints */
+            double tv[ORC_CAP];
+            for (int a = 0; a < n; ++a) tv[a] = nd->e[a].val;
+            for (int i = 0; i < nd_dot; ++i) {
+                if (dot[i].v1 == 0.0) continue;
+                for (int a = 0; a < n; ++a) {
+                    if (nd->e[a].idx == dot[i].a) tv[a] += dot[i].v1 * dt;
+                    if (nd->e[a].idx == dot[i].b) tv[a] += -dot[i].v1 * dt;
+                }
+            }
+            for (int a = 0; a < n; ++a)
+                if (tv[a] < -FLT_EPSILON || tv[a] > 1.0 + FLT_EPSILON) res->warnings++;
+        }
+    }
+
+    /* PhaseField::MergeIncrementsSR (src/PhaseField.cpp:1268-1299) */
+    for (int i = 0; i < nd_dot; ++i) {
+        double value = 1.0 * (dot[i].v1 + dot[i].v2) * dt;
+        if (fabs(value) >= DBL_EPSILON) {
+            if (node_add_value(nd, dot[i].a, value) || node_add_value(nd, dot[i].b, -value)) {
+                res->err = ORC_ERR_CAPACITY;
+                return;
+            }
+        }
+    }
+}
+
+/* NodePF::finalize (include/Containers/NodePF.h:214-255) */
+static int node_finalize(orc_node* nd)
+{
+    double total = 0.0;
+    int w = 0;
+    for (int i = 0; i < nd->n; ++i) {
+        orc_entry e = nd->e[i];
+        int erase = 0;
+        if (e.val >= 1.0 - DBL_EPSILON) e.val = 1.0;
+        if (e.val <= 0.0 + DBL_EPSILON) {
+            e.val = 0.0;
+            erase = 1;
+        }
+        total += e.val;
+        if (!erase) nd->e[w++] = e;
+    }
+    nd->n = w;
+    if (w == 0) return -1;
+    if (w > 1) {
+        double norm = 1.0 / total;
+        for (int i = 0; i < w; ++i) {
+            nd->e[i].val *= norm;
+            nd->e[i].lap = 0.0;
+            nd->e[i].grad[0] = nd->e[i].grad[1] = nd->e[i].grad[2] = 0.0;
+        }
+        nd->flag = 2;
+    } else {
+        nd->e[0].val = 1.0;
+        nd->e[0].lap = 0.0;
+        nd->e[0].grad[0] = nd->e[0].grad[1] = nd->e[0].grad[2] = 0.0;
+        nd->flag = 0;
+    }
+    return 0;
+}
+
+/* PhaseField::FinalizeSR (src/PhaseField.cpp:1008-1033) */
+static int finalize_sr(orc_ctx* c)
+{
+    int err = 0;
+    const long N = c->ncells;
+#pragma omp parallel for schedule(static) reduction(|:err)
+    for (long i = 0; i < N; ++i)
+        if (c->nodes[i].flag != 0)
+            if (node_finalize(&c->nodes[i])) err |= 1;
+    if (err) return ORC_ERR_EMPTY_NODE;
+
+    /* SetFlagsSR (src/PhaseField.cpp:953-969): flag 1 on the 3^d neighbours of flag-2 nodes */
+#pragma omp parallel for schedule(static)
+    for (long ci = 0; ci < N; ++ci) {
+        long i = ci / ((long)c->ny * c->nz), j = (ci / c->nz) % c->ny, k = ci % c->nz;
+        int f = c->nodes[ci].flag;
+        int nf = f;
+        if (f == 0) {
+            for (int ii = -c->dn[0]; ii <= c->dn[0] && !nf; ++ii)
+            for (int jj = -c->dn[1]; jj <= c->dn[1] && !nf; ++jj)
+            for (int kk = -c->dn[2]; kk <= c->dn[2] && !nf; ++kk) {
+                long nb = cell_index(c, wrap(i + ii, c->nx), wrap(j + jj, c->ny), wrap(k + kk, c->nz));
+                if (c->nodes[nb].flag == 2) nf = 1;
+            }
+        }
+        c->tmp[ci].flag = nf;
+    }
+#pragma omp parallel for schedule(static)
+    for (long ci = 0; ci < N; ++ci) c->nodes[ci].flag = c->tmp[ci].flag;
+
+    /* CalculateDerivativesSR (src/PhaseField.cpp:566-615): Laplacian and gradient of every
+     * neighbour entry with a non-zero value, accumulated into a copy of the own list; indices
+     * not yet present are appended with value 0 (the neighbour-list merge). */
+#pragma omp parallel for schedule(static) reduction(|:err)
+    for (long ci = 0; ci < N; ++ci) {
+        orc_node* dst = &c->tmp[ci];
+        const orc_node* src = &c->nodes[ci];
+        *dst = *src;
+        if (src->flag == 0) continue;
+        long i = ci / ((long)c->ny * c->nz), j = (ci / c->nz) % c->ny, k = ci % c->nz;
+        for (int s = 0; s < c->nls; ++s) {
+            const orc_stencil* st = &c->ls[s];
+            long nb = cell_index(c, wrap(i + st->di, c->nx), wrap(j + st->dj, c->ny), wrap(k + st->dk, c->nz));
+            const orc_node* nn = &c->nodes[nb];
+            for (int q = 0; q < nn->n; ++q) {
+                if (nn->e[q].val == 0.0) continue;
+                double lap = st->w * nn->e[q].val;
+                int found = 0;
+                for (int t = 0; t < dst->n; ++t)
+                    if (dst->e[t].idx == nn->e[q].idx) { dst->e[t].lap += lap; found = 1; break; }
+                if (!found) {
+                    if (dst->n >= ORC_CAP) { err |= 1; break; }
+                    orc_entry* e = &dst->e[dst->n++];
+                    e->idx = nn->e[q].idx; e->val = 0.0; e->lap = lap;
+                    e->grad[0] = e->grad[1] = e->grad[2] = 0.0;
+                }
+            }
+        }
+        if (c->need_grad)
+        for (int s = 0; s < c->ngs; ++s) {
+            const orc_stencil* st = &c->gs[s];
+            long nb = cell_index(c, wrap(i + st->di, c->nx), wrap(j + st->dj, c->ny), wrap(k + st->dk, c->nz));
+            const orc_node* nn = &c->nodes[nb];
+            for (int q = 0; q < nn->n; ++q) {
+                if (nn->e[q].val == 0.0) continue;
+                double gx = st->gx * nn->e[q].val, gy = st->gy * nn->e[q].val, gz = st->gz * nn->e[q].val;
+                int found = 0;
+                for (int t = 0; t < dst->n; ++t)
+                    if (dst->e[t].idx == nn->e[q].idx) {
+                        dst->e[t].grad[0] = dst->e[t].grad[0] + gx;
+                        dst->e[t].grad[1] = dst->e[t].grad[1] + gy;
+                        dst->e[t].grad[2] = dst->e[t].grad[2] + gz;
+                        found = 1;
+                        break;
+                    }
+                if (!found) {
+                    if (dst->n >= ORC_CAP) { err |= 1; break; }
+                    orc_entry* e = &dst->e[dst->n++];
+                    e->idx = nn->e[q].idx; e->val = 0.0; e->lap = 0.0;
+                    e->grad[0] = gx; e->grad[1] = gy; e->grad[2] = gz;
+                }
+            }
+        }
+    }
+    if (err) return ORC_ERR_CAPACITY;
+    orc_node* t = c->nodes; c->nodes = c->tmp; c->tmp = t;
+    {
+        int mx = 0, mf = 0;
+        for (long ci = 0; ci < N; ++ci) {
+            const orc_node* nd = &c->nodes[ci];
+            if (nd->n > mx) mx = nd->n;
+            int nz = 0;
+            for (int q = 0; q < nd->n; ++q) nz += nd->e[q].val != 0.0;
+            if (nz > mf) mf = nz;
+        }
+        c->max_list = mx > c->max_list ? mx : c->max_list;
+        c->max_final = mf > c->max_final ? mf : c->max_final;
+    }
+
+    /* CalculateGrainsVolume (src/PhaseField.cpp:812-951), single-thread summation order */
+    for (int g = 0; g < c->ngrains; ++g) c->volume[g] = 0.0;
+    for (long ci = 0; ci < N; ++ci) {
+        const orc_node* nd = &c->nodes[ci];
+        for (int q = 0; q < nd->n; ++q) c->volume[nd->e[q].idx] += nd->e[q].val;
+    }
+    for (int g = 0; g < c->ngrains; ++g) {
+        c->maxvolume[g] = std_max(c->volume[g], c->maxvolume[g]);
+        c->vr[g] = c->maxvolume[g] / c->refvolume[g];
+    }
+    for (int g = 0; g < c->ngrains; ++g) {
+        if (c->exist[g] && c->volume[g] <= 0.0) {
+            c->exist[g] = 0;
+            c->stage[g] = 2;
+            c->volume[g] = 0.0;
+            c->maxvolume[g] = 0.0;
+            c->vr[g] = 1.0;
+        } else if (c->volume[g] > 0.0) {
+            c->exist[g] = 1;
+            if (c->stage[g] == 0) c->stage[g] = 1;
+            if (c->vr[g] > 1.0) {
+                c->stage[g] = 2;
+                c->vr[g] = 1.0;
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+int orc_finalize_initialization(orc_ctx* c)
+{
+    return finalize_sr(c);
+}
+
+int orc_step(orc_ctx* c, double dt, int32_t nsteps)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        double mx_e = 0.0, mx_m = 0.0;
+        int64_t warn = 0;
+        int err = 0;
+        const long N = c->ncells;
+#pragma omp parallel
+        {
+            local_result r = {0.0, 0.0, 0, 0};
+#pragma omp for schedule(static)
+            for (long i = 0; i < N; ++i)
+                if (c->nodes[i].flag != 0) local_step(c, &c->nodes[i], dt, &r);
+#pragma omp critical
+            {
+                mx_e = std_max(mx_e, r.max_energy);
+                mx_m = std_max(mx_m, r.max_mobility);
+                warn += r.warnings;
+                err |= r.err;
+            }
+        }
+        c->max_energy = mx_e;
+        c->max_mobility = mx_m;
+        c->warnings += warn;
+        if (err) return err;
+        int st = finalize_sr(c);
+        if (st) return st;
+    }
+    return ORC_OK;
+}
+
+int64_t orc_total_entries(const orc_ctx* c)
+{
+    int64_t t = 0;
+    for (long i = 0; i < c->ncells; ++i)
+        for (int q = 0; q < c->nodes[i].n; ++q) t += c->nodes[i].e[q].val != 0.0;
+    return t;
+}
+
+int orc_download(const orc_ctx* c, int32_t* count, int32_t* index, double* value, int8_t* flag)
+{
+    long off = 0;
+    for (long i = 0; i < c->ncells; ++i) {
+        const orc_node* nd = &c->nodes[i];
+        int m = 0;
+        for (int q = 0; q < nd->n; ++q) {
+            if (nd->e[q].val == 0.0) continue;
+            if (index) index[off] = (int32_t)nd->e[q].idx;
+            if (value) value[off] = nd->e[q].val;
+            ++off;
+            ++m;
+        }
+        if (count) count[i] = m;
+        if (flag) flag[i] = (int8_t)nd->flag;
+    }
+    return ORC_OK;
+}
+
+int orc_grains(const orc_ctx* c, double* volume, double* volume_ratio)
+{
+    for (int g = 0; g < c->ngrains; ++g) {
+        if (volume) volume[g] = c->volume[g];
+        if (volume_ratio) volume_ratio[g] = c->vr[g];
+    }
+    return ORC_OK;
+}
+
+int64_t orc_limiter_warnings(const orc_ctx* c) { return c->warnings; }
+
+double orc_max_time_step(const orc_ctx* c)
+{
+    double dt = DBL_MAX;
+    if (c->max_energy > DBL_EPSILON && c->max_mobility > 0.0) {
+        double f = c->active == 1 ? 0.50 : (c->active == 2 ? 0.25 : 0.16);
+        dt = f * c->p.dx * c->p.dx / (c->max_mobility * c->max_energy * c->p.regularization_factor);
+    }
+    return dt;
+}
+
+void orc_list_stats(const orc_ctx* c, int32_t* max_merged, int32_t* max_finalized)
+{
+    if (max_merged) *max_merged = c->max_list;
+    if (max_finalized) *max_finalized = c->max_final;
+}

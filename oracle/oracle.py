@@ -1,0 +1,252 @@
+"""ctypes binding of the CPU oracle (libmpf_oracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests, the smoke check and the CPU-baseline leg of bench.py import this module; it mirrors
+openphase_amd.engine.Engine so the same test drives both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libmpf_oracle.so"
+REF_DRIVER = HERE / "_ref" / "ref_driver"
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("dx", C.c_double), ("iwidth", C.c_double),
+        ("nphases", C.c_int32),
+        ("sigma", C.POINTER(C.c_double)), ("mu", C.POINTER(C.c_double)),
+        ("energy_model", C.POINTER(C.c_int32)), ("epsilon", C.POINTER(C.c_double)),
+        ("tj_model", C.c_int32), ("tj_factor", C.c_double), ("tj_energy", C.c_double),
+        ("consider_nucleus_volume", C.c_int32), ("nucleus_volume_factor", C.c_double),
+        ("dg_mode", C.POINTER(C.c_int32)), ("dg_value", C.POINTER(C.c_double)),
+        ("dg_limit", C.c_double), ("regularization_factor", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+        lib = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        i32p, f64p = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        lib.orc_create.argtypes = [C.POINTER(OrcParams), C.c_int32, i32p, f64p, C.POINTER(C.c_uint8), C.POINTER(vp)]
+        lib.orc_destroy.argtypes = [vp]
+        lib.orc_load.argtypes = [vp, i32p, i32p, f64p]
+        lib.orc_finalize_initialization.argtypes = [vp]
+        lib.orc_step.argtypes = [vp, C.c_double, C.c_int32]
+        lib.orc_total_entries.argtypes = [vp]
+        lib.orc_total_entries.restype = C.c_int64
+        lib.orc_download.argtypes = [vp, i32p, i32p, f64p, C.POINTER(C.c_int8)]
+        lib.orc_grains.argtypes = [vp, f64p, f64p]
+        lib.orc_limiter_warnings.argtypes = [vp]
+        lib.orc_limiter_warnings.restype = C.c_int64
+        lib.orc_max_time_step.argtypes = [vp]
+        lib.orc_max_time_step.restype = C.c_double
+        lib.orc_set_threads.argtypes = [C.c_int32]
+        _lib = lib
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class Oracle:
+    """Same surface as openphase_amd.engine.Engine, computed by the C oracle on the CPU."""
+
+    def __init__(self, cfg, threads: int | None = None):
+        from openphase_amd.microstructure import storage_shape  # noqa: F401  (shape helper)
+
+        self.lib = load()
+        if threads:
+            self.lib.orc_set_threads(int(threads))
+        self.cfg = cfg
+        np2 = cfg.nphases * cfg.nphases
+        self._keep = []
+
+        def arr(a, dtype):
+            if a is None:
+                return None
+            x = np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(np2))
+            self._keep.append(x)
+            return x
+
+        p = OrcParams()
+        p.nx, p.ny, p.nz = cfg.nx, cfg.ny, cfg.nz
+        p.dx, p.iwidth = cfg.dx, cfg.iwidth
+        p.nphases = cfg.nphases
+        p.sigma = _p(arr(cfg.sigma, np.float64), C.c_double)
+        p.mu = _p(arr(cfg.mu, np.float64), C.c_double)
+        p.energy_model = _p(arr(cfg.energy_model, np.int32), C.c_int32)
+        p.epsilon = _p(arr(cfg.epsilon, np.float64), C.c_double)
+        p.tj_model, p.tj_factor, p.tj_energy = cfg.tj_model, cfg.tj_factor, cfg.tj_energy
+        p.consider_nucleus_volume = int(cfg.consider_nucleus_volume)
+        p.nucleus_volume_factor = cfg.nucleus_volume_factor
+        p.dg_mode = _p(arr(cfg.dg_mode, np.int32), C.c_int32)
+        p.dg_value = _p(arr(cfg.dg_value, np.float64), C.c_double)
+        p.dg_limit = cfg.dg_limit
+        p.regularization_factor = cfg.regularization_factor
+        self._p = p
+        self.h = None
+        self.ngrains = 0
+
+    def set_grains(self, phase, rotation=None, solid=None):
+        phase = np.ascontiguousarray(phase, dtype=np.int32)
+        rot = None if rotation is None else np.ascontiguousarray(rotation, dtype=np.float64).reshape(-1)
+        sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8)
+        self._grain_args = (phase, rot, sol)
+        if self.h:
+            self.lib.orc_destroy(self.h)
+        h = C.c_void_p()
+        st = self.lib.orc_create(C.byref(self._p), phase.shape[0], _p(phase, C.c_int32), _p(rot, C.c_double),
+                                 _p(sol, C.c_uint8), C.byref(h))
+        if st:
+            raise RuntimeError(f"orc_create failed: {st}")
+        self.h = h
+        self.ngrains = phase.shape[0]
+
+    def upload(self, lists):
+        cnt = np.ascontiguousarray(lists.count, dtype=np.int32)
+        idx = np.ascontiguousarray(lists.index, dtype=np.int32)
+        val = np.ascontiguousarray(lists.value, dtype=np.float64)
+        st = self.lib.orc_load(self.h, _p(cnt, C.c_int32), _p(idx, C.c_int32), _p(val, C.c_double))
+        if st:
+            raise RuntimeError(f"orc_load failed: {st}")
+
+    def finalize_initialization(self):
+        st = self.lib.orc_finalize_initialization(self.h)
+        if st:
+            raise RuntimeError(f"orc_finalize_initialization failed: {st}")
+
+    def step(self, dt, nsteps=1):
+        st = self.lib.orc_step(self.h, float(dt), int(nsteps))
+        if st:
+            raise RuntimeError(f"orc_step failed: {st}")
+
+    def download(self, with_flags=False):
+        from openphase_amd.microstructure import Lists
+
+        n = int(np.prod(self.cfg.shape))
+        tot = int(self.lib.orc_total_entries(self.h))
+        cnt = np.zeros(n, np.int32)
+        idx = np.zeros(tot, np.int32)
+        val = np.zeros(tot, np.float64)
+        flg = np.zeros(n, np.int8)
+        self.lib.orc_download(self.h, _p(cnt, C.c_int32), _p(idx, C.c_int32), _p(val, C.c_double), _p(flg, C.c_int8))
+        lists = Lists(self.cfg.shape, cnt, idx, val)
+        return (lists, flg) if with_flags else lists
+
+    def grains(self):
+        vol = np.zeros(self.ngrains)
+        vr = np.zeros(self.ngrains)
+        self.lib.orc_grains(self.h, _p(vol, C.c_double), _p(vr, C.c_double))
+        return vol, vr
+
+    def limiter_warnings(self):
+        return int(self.lib.orc_limiter_warnings(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_opi(path, cfg, threads: int = 1) -> None:
+    """Input file for the reference driver with the keys the reference modules read
+    (GridParameters, Settings, InterfaceProperties, PhaseField, BoundaryConditions,
+    UserDrivingForce, DrivingForce)."""
+    lines = ["Generated input", "", "@RunTimeControl", f"$nOMP : {threads}", "",
+             "@GridParameters", f"$Nx : {cfg.nx}", f"$Ny : {cfg.ny}", f"$Nz : {cfg.nz}",
+             f"$dx : {cfg.dx!r}", f"$IWidth : {cfg.iwidth!r}", "", "@Settings"]
+    for n in range(cfg.nphases):
+        lines.append(f"$Phase_{n} : Phase{n}")
+    lines += ["", "@PhaseField",
+              f"$ConsiderNucleusVolume : {'Yes' if cfg.consider_nucleus_volume else 'No'}",
+              f"$NucleusVolumeFactor : {cfg.nucleus_volume_factor!r}", "", "@InterfaceProperties"]
+    sig = np.asarray(cfg.sigma, dtype=np.float64).reshape(cfg.nphases, cfg.nphases)
+    mu = np.asarray(cfg.mu, dtype=np.float64).reshape(cfg.nphases, cfg.nphases)
+    em = None if cfg.energy_model is None else np.asarray(cfg.energy_model).reshape(cfg.nphases, cfg.nphases)
+    eps = None if cfg.epsilon is None else np.asarray(cfg.epsilon).reshape(cfg.nphases, cfg.nphases)
+    for a in range(cfg.nphases):
+        for b in range(a, cfg.nphases):
+            if em is not None and em[a, b] == 1:
+                lines.append(f"$EnergyModel_{a}_{b} : CUBIC")
+                lines.append(f"$EpsilonE_{a}_{b} : {float(eps[a, b])!r}")
+            else:
+                lines.append(f"$EnergyModel_{a}_{b} : ISO")
+            lines.append(f"$Sigma_{a}_{b} : {float(sig[a, b])!r}")
+            lines.append(f"$MobilityModel_{a}_{b} : ISO")
+            lines.append(f"$Mu_{a}_{b} : {float(mu[a, b])!r}")
+    tj = {0: "MODEL", 1: "VALUE", 2: "NONE"}[cfg.tj_model]
+    lines.append(f"$TripleJunctionModel : {tj}")
+    if tj == "MODEL":
+        lines.append(f"$TripleJunctionFactor : {cfg.tj_factor!r}")
+    if tj == "VALUE":
+        lines.append(f"$TripleJunctionEnergy : {cfg.tj_energy!r}")
+    lines.append(f"$RegularizationFactor : {cfg.regularization_factor!r}")
+    lines += ["", "@BoundaryConditions"]
+    for ax in "XYZ":
+        lines += [f"$BC0{ax} : Periodic", f"$BCN{ax} : Periodic"]
+    if cfg.dg_mode is not None:
+        mode = np.asarray(cfg.dg_mode).reshape(cfg.nphases, cfg.nphases)
+        val = np.asarray(cfg.dg_value, dtype=np.float64).reshape(cfg.nphases, cfg.nphases)
+        lines += ["", "@UserDrivingForce"]
+        for a in range(cfg.nphases):
+            for b in range(a + 1, cfg.nphases):
+                if mode[a, b] == 1:
+                    lines += [f"$UDF_Mode_{a}_{b} : VALUE", f"$UDF_Value_{a}_{b} : {float(val[a, b])!r}"]
+        lines += ["", "@DrivingForce", "$Average : No"]
+        for a in range(cfg.nphases):
+            for b in range(a, cfg.nphases):
+                lines.append(f"$Limit_{a}_{b} : {cfg.dg_limit!r}")
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dumps, threads=1, timeout=3600):
+    """Runs the real reference (oracle/_ref/ref_driver) on `lists`; returns the parsed timing JSON
+    and {step: (Lists, volume, volume_ratio)} for the dump steps."""
+    import json
+
+    from openphase_amd import rawio
+
+    workdir = Path(workdir)
+    workdir.mkdir(parents=True, exist_ok=True)
+    opi = workdir / "input.opi"
+    write_opi(opi, cfg, threads)
+    rawio.write_phasefield(str(workdir / "state.dat"), lists)
+    rawio.write_grains(str(workdir / "grains.bin"), grain_phase, quaternions)
+    cmd = [str(REF_DRIVER), str(opi), str(workdir / "state.dat"), str(workdir / "grains.bin"), str(nsteps),
+           repr(float(dt)), str(workdir / "out"), ",".join(str(s) for s in dumps), "--threads", str(threads)]
+    if cfg.dg_mode is not None:
+        cmd.append("--driving-force")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env, cwd=workdir)
+    if res.returncode != 0:
+        raise RuntimeError(f"ref_driver failed ({res.returncode}): {res.stdout[-2000:]} {res.stderr[-2000:]}")
+    timing = json.loads([ln for ln in res.stdout.splitlines() if ln.startswith("{\"cells\"")][-1])
+    out = {}
+    ng = len(grain_phase)
+    for s in dumps:
+        st = rawio.read_phasefield(str(workdir / f"out_{s}.dat"))
+        vol = np.fromfile(workdir / f"out_{s}.vol", dtype=np.float64)
+        out[s] = (st, vol[:ng], vol[ng:2 * ng])
+    return timing, out

@@ -1,0 +1,161 @@
+// Golden-vector driver for the real OpenPhase library (test infrastructure).
+//
+// Built only by oracle/Makefile.ref against the reference sources under
+// /root/reference (compiled from source into oracle/_ref/, never copied).
+// It runs the canonical grain-growth loop of examples/NormalGG/NormalGG.cpp:74-85
+//   IP.Set -> DO.CalculatePhaseFieldIncrements -> Phi.NormalizeIncrements -> Phi.MergeIncrements
+// (optionally with the driving-force calls of benchmarks/AnisotropyTest/AnisotropyTest.cpp:140-155)
+// on a microstructure read from a raw PhaseField .dat file (src/PhaseField.cpp:2340-2388 format)
+// and writes the finalized state at selected steps, in the same raw format but without the
+// zero-valued entries that CalculateDerivativesSR appends (src/PhaseField.cpp:566-615).
+//
+// usage: ref_driver <input.opi> <state.dat> <grains.bin> <nsteps> <dt> <outprefix> <dumpsteps,...>
+//        [--driving-force] [--threads N]
+// grains.bin: uint32 n; n x { uint32 phase; double q[4] }   (q = s, x, y, z)
+// Writes <outprefix>_<step>.dat and <outprefix>_<step>.vol (n doubles Volume, n doubles VolumeRatio)
+// and prints one JSON line with per-call wall times.
+
+#include "Settings.h"
+#include "RunTimeControl.h"
+#include "DoubleObstacle.h"
+#include "PhaseField.h"
+#include "BoundaryConditions.h"
+#include "InterfaceProperties.h"
+#include "DrivingForce.h"
+#include "UserDrivingForce.h"
+
+#include <cstdio>
+#include <cstdint>
+#include <fstream>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+using namespace openphase;
+
+static void dump_state(const PhaseField& Phi, const std::string& prefix, int step)
+{
+    std::string fn = prefix + "_" + std::to_string(step) + ".dat";
+    std::ofstream out(fn, std::ios::binary);
+    int nx = Phi.Grid.Nx, ny = Phi.Grid.Ny, nz = Phi.Grid.Nz;
+    out.write(reinterpret_cast<const char*>(&nx), sizeof(int));
+    out.write(reinterpret_cast<const char*>(&ny), sizeof(int));
+    out.write(reinterpret_cast<const char*>(&nz), sizeof(int));
+    for (long i = 0; i < nx; ++i)
+    for (long j = 0; j < ny; ++j)
+    for (long k = 0; k < nz; ++k)
+    {
+        const NodePF& node = Phi.Fields(i, j, k);
+        size_t n = 0;
+        for (auto it = node.cbegin(); it != node.cend(); ++it) if (it->value != 0.0) ++n;
+        out.write(reinterpret_cast<const char*>(&n), sizeof(size_t));
+        for (auto it = node.cbegin(); it != node.cend(); ++it)
+        if (it->value != 0.0)
+        {
+            size_t idx = it->index;
+            double v = it->value;
+            out.write(reinterpret_cast<const char*>(&idx), sizeof(size_t));
+            out.write(reinterpret_cast<const char*>(&v), sizeof(double));
+        }
+    }
+    std::string fv = prefix + "_" + std::to_string(step) + ".vol";
+    std::ofstream ov(fv, std::ios::binary);
+    size_t ng = Phi.FieldsProperties.size();
+    for (size_t g = 0; g < ng; ++g) ov.write(reinterpret_cast<const char*>(&Phi.FieldsProperties[g].Volume), sizeof(double));
+    for (size_t g = 0; g < ng; ++g) ov.write(reinterpret_cast<const char*>(&Phi.FieldsProperties[g].VolumeRatio), sizeof(double));
+}
+
+int main(int argc, char* argv[])
+{
+    if (argc < 8)
+    {
+        std::fprintf(stderr, "usage: %s input.opi state.dat grains.bin nsteps dt outprefix dumpsteps [--driving-force] [--threads N]\n", argv[0]);
+        return 2;
+    }
+    std::string opi = argv[1], state = argv[2], grains = argv[3];
+    int nsteps = std::atoi(argv[4]);
+    double dt = std::strtod(argv[5], nullptr);
+    std::string prefix = argv[6];
+    std::set<int> dumps;
+    {
+        std::stringstream ss(argv[7]);
+        std::string tok;
+        while (std::getline(ss, tok, ',')) if (!tok.empty()) dumps.insert(std::atoi(tok.c_str()));
+    }
+    bool with_dg = false;
+    int threads = 1;
+    for (int a = 8; a < argc; ++a)
+    {
+        std::string s = argv[a];
+        if (s == "--driving-force") with_dg = true;
+        if (s == "--threads" && a + 1 < argc) threads = std::atoi(argv[++a]);
+    }
+    omp_set_num_threads(threads);
+
+    Settings OPSettings;
+    OPSettings.ReadInput(opi);
+    PhaseField Phi(OPSettings, opi);
+    DoubleObstacle DO(OPSettings, opi);
+    InterfaceProperties IP(OPSettings, opi);
+    BoundaryConditions BC(OPSettings, opi);
+    DrivingForce* dG = nullptr;
+    UserDrivingForce* UDF = nullptr;
+    if (with_dg)
+    {
+        dG = new DrivingForce(OPSettings, opi);
+        UDF = new UserDrivingForce(OPSettings, opi);
+    }
+
+    std::ifstream gin(grains, std::ios::binary);
+    uint32_t ng = 0;
+    gin.read(reinterpret_cast<char*>(&ng), sizeof(uint32_t));
+    for (uint32_t g = 0; g < ng; ++g)
+    {
+        uint32_t phase = 0;
+        double q[4];
+        gin.read(reinterpret_cast<char*>(&phase), sizeof(uint32_t));
+        gin.read(reinterpret_cast<char*>(q), 4 * sizeof(double));
+        size_t idx = Phi.AddGrainInfo(phase);
+        Phi.FieldsProperties[idx].Orientation.set(q[0], q[1], q[2], q[3]);
+    }
+    if (!Phi.Read(state)) return 3;
+    Phi.FinalizeInitialization(BC);
+    if (dumps.count(0)) dump_state(Phi, prefix, 0);
+
+    double t_ip = 0, t_do = 0, t_norm = 0, t_merge = 0, t_dg = 0;
+    for (int step = 1; step <= nsteps; ++step)
+    {
+        double t0 = omp_get_wtime();
+        if (with_dg) dG->Clear();
+        IP.Set(Phi, BC);
+        double t1 = omp_get_wtime();
+        if (with_dg) DO.CalculatePhaseFieldIncrements(Phi, IP, *dG);
+        else         DO.CalculatePhaseFieldIncrements(Phi, IP);
+        double t2 = omp_get_wtime();
+        if (with_dg)
+        {
+            UDF->SetDrivingForce(Phi, *dG);
+            dG->Average(Phi, BC);
+            dG->MergePhaseFieldIncrements(Phi, IP);
+        }
+        double t3 = omp_get_wtime();
+        Phi.NormalizeIncrements(BC, dt);
+        double t4 = omp_get_wtime();
+        Phi.MergeIncrements(BC, dt);
+        double t5 = omp_get_wtime();
+        t_ip += t1 - t0; t_do += t2 - t1; t_dg += t3 - t2; t_norm += t4 - t3; t_merge += t5 - t4;
+        if (dumps.count(step)) dump_state(Phi, prefix, step);
+    }
+    long cells = (long)Phi.Grid.Nx * Phi.Grid.Ny * Phi.Grid.Nz;
+    double total = t_ip + t_do + t_dg + t_norm + t_merge;
+    std::printf("{\"cells\": %ld, \"steps\": %d, \"threads\": %d, \"IP.Set\": %.6f, \"CalculatePhaseFieldIncrements\": %.6f, "
+                "\"DrivingForce\": %.6f, \"NormalizeIncrements\": %.6f, \"MergeIncrements\": %.6f, \"total_s\": %.6f, "
+                "\"cell_updates_per_s\": %.6e}\n",
+                cells, nsteps, threads, t_ip, t_do, t_dg, t_norm, t_merge, total,
+                total > 0 ? double(cells) * nsteps / total : 0.0);
+    delete dG;
+    delete UDF;
+    return 0;
+}

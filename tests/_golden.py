@@ -1,0 +1,57 @@
+"""Loading of the golden fixtures written by tests/golden/make_golden.py."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+from openphase_amd.engine import StepConfig
+from openphase_amd.microstructure import Lists, quaternion_to_matrix, storage_shape
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+CASES = sorted(p.stem for p in GOLDEN.glob("*.npz"))
+
+
+class Case:
+    def __init__(self, name: str, precision: int = 0):
+        z = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)
+        self.name = name
+        nph = int(z["nphases"])
+        self.cfg = StepConfig(
+            int(z["nx"]), int(z["ny"]), int(z["nz"]), dx=float(z["dx"]), iwidth=float(z["iwidth"]),
+            nphases=nph, sigma=z["sigma"], mu=z["mu"],
+            energy_model=z["energy_model"] if z["energy_model"].any() else None,
+            epsilon=z["epsilon"] if z["energy_model"].any() else None,
+            tj_model=int(z["tj_model"]), tj_factor=float(z["tj_factor"]), tj_energy=float(z["tj_energy"]),
+            dg_mode=z["dg_mode"] if bool(z["has_dg"]) else None,
+            dg_value=z["dg_value"] if bool(z["has_dg"]) else None,
+            precision=precision,
+            slots=16,  # the reference's lists are unbounded; the densest fixture needs 9
+        )
+        self.dt = float(z["dt"])
+        self.phase = z["grain_phase"].astype(np.int32)
+        q = z["quaternions"]
+        self.rotation = quaternion_to_matrix(q) if np.any(q) else None
+        shape = storage_shape(self.cfg.nx, self.cfg.ny, self.cfg.nz)
+        self.init = Lists(shape, z["init_count"], z["init_index"], z["init_value"])
+        self.dumps = [int(s) for s in z["dumps"]]
+        self.expect = {
+            s: (Lists(shape, z[f"s{s}_count"], z[f"s{s}_index"], z[f"s{s}_value"]), z[f"s{s}_volume"], z[f"s{s}_vr"])
+            for s in self.dumps
+        }
+
+    def setup(self, engine):
+        engine.set_grains(self.phase, self.rotation)
+        engine.upload(self.init)
+        engine.finalize_initialization()
+        return engine
+
+    def run(self, engine):
+        """Yields (step, lists, volume, volume_ratio) at every dump step."""
+        done = 0
+        for s in self.dumps:
+            if s > done:
+                engine.step(self.dt, s - done)
+                done = s
+            vol, vr = engine.grains()
+            yield s, engine.download(), vol, vr

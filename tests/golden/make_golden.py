@@ -1,0 +1,91 @@
+"""Writes the golden fixtures of tests/golden/*.npz from the real OpenPhase code.
+
+Runs oracle/_ref/ref_driver (built from the reference sources by oracle/Makefile.ref) on small
+seeded microstructures and stores, per case, the input state, the parameters and the finalized
+state (lists, grain volumes, volume ratios) at a few steps. The tests compare the CPU oracle and
+the GPU path against these files; the reference itself is never needed at test time.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+from openphase_amd.engine import StepConfig  # noqa: E402
+from openphase_amd.microstructure import random_rotations, voronoi_grains  # noqa: E402
+import oracle as O  # noqa: E402
+
+HERE = Path(__file__).resolve().parent
+
+
+def cases():
+    """(name, cfg, ngrains, seed, grain_phase, quaternions, nsteps, dumps, dt_factor)."""
+    out = []
+    # cfg1-like: 2D isotropic grain growth, single phase
+    out.append(("gg2d_32", StepConfig(32, 32, 0), 8, 11, None, None, 40, [0, 1, 2, 5, 20, 40], 1.0))
+    # 3D isotropic grain growth, 27-point stencil
+    out.append(("gg3d_16", StepConfig(16, 16, 16), 10, 12, None, None, 20, [0, 1, 3, 10, 20], 1.0))
+    # grains below the reference volume: VolumeRatio < 1 and the triple-junction term
+    out.append(("small2d_24", StepConfig(24, 24, 0), 30, 13, None, None, 30, [0, 1, 5, 30], 1.0))
+    # anisotropic box, 3D, uneven sizes
+    out.append(("box3d", StepConfig(12, 18, 10), 14, 14, None, None, 15, [0, 1, 5, 15], 1.0))
+    # two phases with different energies/mobilities, TJ VALUE model
+    two = StepConfig(32, 32, 0, nphases=2,
+                     sigma=np.array([[0.24, 0.30], [0.30, 0.20]]),
+                     mu=np.array([[4e-9, 2e-9], [2e-9, 3e-9]]),
+                     tj_model=1, tj_energy=0.05)
+    phase2 = (np.arange(12) % 2).astype(np.int32)
+    out.append(("twophase2d_32", two, 12, 15, phase2, None, 30, [0, 1, 10, 30], 0.9))
+    # cubic anisotropy + constant driving force between two phases (adjoint configuration, small)
+    q, _ = random_rotations(12, 16)
+    aniso = StepConfig(32, 32, 0, nphases=2,
+                       sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                       energy_model=np.ones((2, 2), np.int32), epsilon=np.full((2, 2), 0.05),
+                       dg_mode=np.array([[0, 1], [1, 0]], np.int32),
+                       dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]))
+    out.append(("aniso_dg2d_32", aniso, 12, 16, phase2, q, 30, [0, 1, 10, 30], 0.9))
+    return out
+
+
+def main():
+    for name, cfg, ng, seed, phase, quat, nsteps, dumps, dtf in cases():
+        lists = voronoi_grains(cfg.nx, cfg.ny, cfg.nz, ng, seed=seed)
+        phase = np.zeros(ng, np.int32) if phase is None else phase
+        dt = cfg.max_time_step() * dtf
+        with tempfile.TemporaryDirectory() as td:
+            timing, ref = O.run_reference(td, cfg, lists, phase, quat, nsteps, dt, dumps)
+        rec = dict(
+            nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, dx=cfg.dx, iwidth=cfg.iwidth, nphases=cfg.nphases,
+            sigma=np.asarray(cfg.sigma, np.float64), mu=np.asarray(cfg.mu, np.float64),
+            energy_model=np.zeros((cfg.nphases, cfg.nphases), np.int32) if cfg.energy_model is None
+            else np.asarray(cfg.energy_model, np.int32),
+            epsilon=np.zeros((cfg.nphases, cfg.nphases)) if cfg.epsilon is None else np.asarray(cfg.epsilon),
+            tj_model=cfg.tj_model, tj_factor=cfg.tj_factor, tj_energy=cfg.tj_energy,
+            has_dg=cfg.dg_mode is not None,
+            dg_mode=np.zeros((cfg.nphases, cfg.nphases), np.int32) if cfg.dg_mode is None else cfg.dg_mode,
+            dg_value=np.zeros((cfg.nphases, cfg.nphases)) if cfg.dg_value is None else cfg.dg_value,
+            dt=dt, grain_phase=phase, quaternions=np.zeros((ng, 4)) if quat is None else quat,
+            init_count=lists.count, init_index=lists.index, init_value=lists.value,
+            dumps=np.asarray(dumps, np.int32),
+        )
+        for s in dumps:
+            st, vol, vr = ref[s]
+            rec[f"s{s}_count"] = st.count
+            rec[f"s{s}_index"] = st.index
+            rec[f"s{s}_value"] = st.value
+            rec[f"s{s}_volume"] = vol
+            rec[f"s{s}_vr"] = vr
+        np.savez_compressed(HERE / f"{name}.npz", **rec)
+        print(name, "cells", lists.ncells, "entries", int(lists.count.sum()), "ref s", timing["total_s"])
+
+
+if __name__ == "__main__":
+    main()
