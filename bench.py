@@ -74,7 +74,11 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
     eng.enable_timing(True)
     eng.step(dt, steps)
     k_step_ms, k_grain_ms = eng.last_timing()
+    d_mid, d_wide = eng.step_stats()
+    total_steps = warmup + 2 * steps
     eng.enable_timing(False)
+    copy_bytes = 1 << 30
+    copy_ms = eng.copy_bandwidth(copy_bytes, 10)
     eng.close()
     vbytes = 8 if precision == 0 else 4
     b_alg = algorithmic_bytes(c_in, c_out, vbytes)
@@ -87,6 +91,8 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
         "bytes_per_step": b_alg,
         "mean_entries_per_cell": float((c_in.sum() + c_out.sum()) / (2 * ncell)),
         "achieved_gbs": b_alg / (k_step_ms * 1e-3) / 1e9 if k_step_ms > 0 else None,
+        "copy_gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9,
+        "deferred_cells_per_step": ((d_mid - d_wide) / total_steps, d_wide / total_steps),
     }
 
 
@@ -183,6 +189,7 @@ def main():
             "fp64_mean_entries_per_cell": r64["mean_entries_per_cell"],
             "fp32_step_kernel_ms": r32["kernel_ms"],
             "fp32_bytes_per_step": r32["bytes_per_step"],
+            "copy_kernel_gbs": r64["copy_gbs"],
         },
         "cpu_baseline": cpu,
     }

@@ -154,6 +154,17 @@ mpf_status mpf_last_timing(mpf_ctx* ctx, double* step_kernel_ms, double* grain_k
 /* Enables/disables per-kernel HIP event timing in mpf_step (off by default). */
 mpf_status mpf_enable_timing(mpf_ctx* ctx, int32_t on);
 
+/* Cumulative number of cell-steps finished outside the main kernel since creation: merged lists
+ * of 5-6 entries (or more than 8 own entries) and lists of 1, 7-16 entries (no reference
+ * counterpart: the reference has one code path). */
+mpf_status mpf_step_stats(mpf_ctx* ctx, int64_t* deferred_mid, int64_t* deferred_wide);
+
+/* Measurement helper (no reference counterpart): copies `bytes` (multiple of 16) between two
+ * device buffers owned by the context with a plain streaming copy kernel `repeats` times and
+ * returns the average kernel time in ms; the HBM bandwidth reference for the roofline and the
+ * calibration kernel for rocprof FETCH_SIZE/WRITE_SIZE. */
+mpf_status mpf_copy_bandwidth(mpf_ctx* ctx, int64_t bytes, int32_t repeats, double* ms);
+
 /* Human-readable description of the last error on this context (static storage per context). */
 const char* mpf_error_message(mpf_ctx* ctx);
 

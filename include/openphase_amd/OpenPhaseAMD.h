@@ -1,0 +1,528 @@
+// OpenPhaseAMD.h -- OpenPhase's DoubleObstacle / PhaseField / InterfaceProperties /
+// BoundaryConditions call surface on top of the MI355X step (C ABI include/mpf.h).
+//
+// A grain-growth driver written against the reference (examples/NormalGG/NormalGG.cpp:58-85)
+//
+//     Settings OPSettings;  OPSettings.ReadInput(InputFile);
+//     PhaseField Phi(OPSettings, InputFile);          DoubleObstacle DO(OPSettings, InputFile);
+//     InterfaceProperties IP(OPSettings, InputFile);  BoundaryConditions BC(OPSettings, InputFile);
+//     ... Phi.Read(file) / grains ... ; Phi.FinalizeInitialization(BC);
+//     for (...) { IP.Set(Phi, BC); DO.CalculatePhaseFieldIncrements(Phi, IP);
+//                 Phi.NormalizeIncrements(BC, dt); Phi.MergeIncrements(BC, dt); }
+//
+// compiles unchanged against this header (namespace openphase_amd instead of openphase). The
+// four calls of a step are recorded and executed as one fused device step when MergeIncrements
+// is called; calling them out of order, or asking for the intermediate increments, is an error.
+// Errors are reported by exceptions (std::runtime_error); nothing calls exit().
+//
+// Input: the same .opi files (keys of @GridParameters, @Settings, @PhaseField,
+// @InterfaceProperties, @BoundaryConditions, @UserDrivingForce, @DrivingForce read by the
+// reference's FileInterface, src/FileInterface.cpp:129-345).
+#ifndef OPENPHASE_AMD_OPENPHASEAMD_H
+#define OPENPHASE_AMD_OPENPHASEAMD_H
+
+#include <algorithm>
+#include <cctype>
+#include <cstdint>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../mpf.h"
+
+namespace openphase_amd {
+
+// ------------------------------------------------------------------------------------------
+// .opi reader: "@Module" opens a block, "$Key  free text  : value" defines a key in it.
+class InputFile {
+ public:
+    explicit InputFile(const std::string& path)
+    {
+        std::ifstream in(path);
+        if (!in) throw std::runtime_error("input file \"" + path + "\" could not be opened");
+        std::string line, module;
+        while (std::getline(in, line)) {
+            size_t p = line.find_first_not_of(" \t\r");
+            if (p == std::string::npos) continue;
+            if (line[p] == '@') {
+                std::istringstream ss(line.substr(p + 1));
+                ss >> module;
+                continue;
+            }
+            if (line[p] != '$') continue;
+            std::istringstream ss(line.substr(p + 1));
+            std::string key;
+            ss >> key;
+            size_t colon = line.find(':', p);
+            if (colon == std::string::npos) continue;
+            std::string value = line.substr(colon + 1);
+            size_t a = value.find_first_not_of(" \t\r");
+            size_t b = value.find_last_not_of(" \t\r");
+            value = a == std::string::npos ? std::string() : value.substr(a, b - a + 1);
+            if (!entries_[module].count(key)) entries_[module][key] = value;
+        }
+    }
+    bool has(const std::string& module, const std::string& key) const
+    {
+        auto m = entries_.find(module);
+        return m != entries_.end() && m->second.count(key);
+    }
+    std::string get(const std::string& module, const std::string& key, const std::string& def = std::string(),
+                    bool mandatory = false) const
+    {
+        if (has(module, key)) return entries_.at(module).at(key);
+        if (mandatory) throw std::runtime_error("missing $" + key + " in @" + module);
+        return def;
+    }
+    double getd(const std::string& module, const std::string& key, double def, bool mandatory = false) const
+    {
+        std::string v = get(module, key, std::string(), mandatory);
+        return v.empty() ? def : std::stod(v);
+    }
+    int geti(const std::string& module, const std::string& key, int def, bool mandatory = false) const
+    {
+        std::string v = get(module, key, std::string(), mandatory);
+        return v.empty() ? def : std::stoi(v);
+    }
+    bool getb(const std::string& module, const std::string& key, bool def) const
+    {
+        std::string v = get(module, key);
+        if (v.empty()) return def;
+        for (auto& ch : v) ch = static_cast<char>(std::toupper(static_cast<unsigned char>(ch)));
+        return v == "YES" || v == "TRUE" || v == "1";
+    }
+
+ private:
+    std::map<std::string, std::map<std::string, std::string>> entries_;
+};
+
+inline void check(mpf_status s, mpf_ctx* ctx, const char* where)
+{
+    if (s == MPF_OK) return;
+    std::string msg = std::string(where) + " failed (status " + std::to_string(static_cast<int>(s)) + ")";
+    if (ctx) msg += ": " + std::string(mpf_error_message(ctx));
+    throw std::runtime_error(msg);
+}
+
+// ------------------------------------------------------------------------------------------
+// GridParameters (include/GridParameters.h:37-172) and Settings (include/Settings.h:44-119)
+struct GridParameters {
+    int Nx = 0, Ny = 0, Nz = 0;   // as given in the input (0 = reduced axis)
+    double dx = 1.0;
+    double iWidth = 5.0;
+    double Eta() const { return iWidth * dx; }
+    int Active() const { return (Nx != 0) + (Ny != 0) + (Nz != 0); }
+    long TotalNumberOfCells() const
+    {
+        return static_cast<long>(Nx ? Nx : 1) * (Ny ? Ny : 1) * (Nz ? Nz : 1);
+    }
+};
+
+class Settings {
+ public:
+    void ReadInput(const std::string& file)
+    {
+        InputFile in(file);
+        Grid.Nx = in.geti("GridParameters", "Nx", 0, true);
+        Grid.Ny = in.geti("GridParameters", "Ny", 0, true);
+        Grid.Nz = in.geti("GridParameters", "Nz", 0, true);
+        Grid.dx = in.getd("GridParameters", "dx", 0.0, true);
+        Grid.iWidth = in.has("GridParameters", "IWidth") ? in.getd("GridParameters", "IWidth", 5.0)
+                                                          : in.getd("GridParameters", "InterfaceWidth", 5.0, true);
+        PhaseNames.clear();
+        for (size_t n = 0; in.has("Settings", "Phase_" + std::to_string(n)); ++n)
+            PhaseNames.push_back(in.get("Settings", "Phase_" + std::to_string(n)));
+        Nphases = PhaseNames.size();
+        Solid.assign(Nphases, 1);
+        for (size_t n = 0; n < Nphases; ++n)
+            Solid[n] = in.get("Settings", "State_" + std::to_string(n), "SOLID") == "SOLID" ? 1 : 0;
+        InputFileName = file;
+    }
+    GridParameters Grid;
+    size_t Nphases = 0;
+    std::vector<std::string> PhaseNames;
+    std::vector<uint8_t> Solid;
+    std::string InputFileName;
+};
+
+// ------------------------------------------------------------------------------------------
+class BoundaryConditions {
+ public:
+    BoundaryConditions() = default;
+    BoundaryConditions(Settings& s, const std::string& file) { ReadInput(file); (void)s; }
+    void ReadInput(const std::string& file)
+    {
+        InputFile in(file);
+        const char* keys[6] = {"BC0X", "BCNX", "BC0Y", "BCNY", "BC0Z", "BCNZ"};
+        for (int i = 0; i < 6; ++i) {
+            std::string v = in.get("BoundaryConditions", keys[i], "Periodic");
+            if (v == "Periodic") bc[i] = MPF_BC_PERIODIC;
+            else if (v == "NoFlux") bc[i] = MPF_BC_NOFLUX;
+            else if (v == "Free") bc[i] = MPF_BC_FREE;
+            else if (v == "Fixed") bc[i] = MPF_BC_FIXED;
+            else if (v == "Mirror") bc[i] = MPF_BC_MIRROR;
+            else throw std::runtime_error("unknown boundary condition " + v);
+        }
+    }
+    int32_t bc[6] = {0, 0, 0, 0, 0, 0};
+};
+
+class InterfaceProperties;
+class DoubleObstacle;
+class DrivingForce;
+class UserDrivingForce;
+
+// ------------------------------------------------------------------------------------------
+// PhaseField (include/PhaseField.h): owns the device state through an mpf_ctx.
+class PhaseField {
+ public:
+    PhaseField(Settings& s, const std::string& file, int precision = MPF_FP64, int slots = 8)
+        : Grid(s.Grid), Nphases(s.Nphases), settings_(s), file_(file), precision_(precision), slots_(slots)
+    {
+        InputFile in(file);
+        ConsiderNucleusVolume = in.getb("PhaseField", "ConsiderNucleusVolume", true);
+        NucleusVolumeFactor = in.getd("PhaseField", "NucleusVolumeFactor", 1.0);
+    }
+    ~PhaseField()
+    {
+        if (ctx_) mpf_destroy(ctx_);
+    }
+    PhaseField(const PhaseField&) = delete;
+    PhaseField& operator=(const PhaseField&) = delete;
+
+    // PhaseField::AddGrainInfo (src/PhaseField.cpp:2970-2979)
+    size_t AddGrainInfo(size_t PhaseIndex, const double* rotation9 = nullptr)
+    {
+        grain_phase_.push_back(static_cast<int32_t>(PhaseIndex));
+        for (int q = 0; q < 9; ++q) grain_rot_.push_back(rotation9 ? rotation9[q] : (q % 4 == 0 ? 1.0 : 0.0));
+        grains_dirty_ = true;
+        return grain_phase_.size() - 1;
+    }
+    size_t NumberOfGrains() const { return grain_phase_.size(); }
+
+    // PhaseField::Read(FileName) (src/PhaseField.cpp:2487-2543): raw .dat
+    bool Read(const std::string& FileName)
+    {
+        std::ifstream inp(FileName, std::ios::binary);
+        if (!inp) return false;
+        int n[3];
+        inp.read(reinterpret_cast<char*>(n), sizeof(n));
+        const int sx = Grid.Nx ? Grid.Nx : 1, sy = Grid.Ny ? Grid.Ny : 1, sz = Grid.Nz ? Grid.Nz : 1;
+        if (n[0] != sx || n[1] != sy || n[2] != sz) return false;
+        const long N = static_cast<long>(sx) * sy * sz;
+        std::vector<int32_t> count(N), index;
+        std::vector<double> value;
+        for (long c = 0; c < N; ++c) {
+            uint64_t k = 0;
+            inp.read(reinterpret_cast<char*>(&k), sizeof(k));
+            count[c] = static_cast<int32_t>(k);
+            for (uint64_t q = 0; q < k; ++q) {
+                uint64_t idx = 0;
+                double v = 0.0;
+                inp.read(reinterpret_cast<char*>(&idx), sizeof(idx));
+                inp.read(reinterpret_cast<char*>(&v), sizeof(v));
+                index.push_back(static_cast<int32_t>(idx));
+                value.push_back(v);
+            }
+        }
+        if (!inp) return false;
+        Upload(count.data(), index.data(), value.data());
+        return true;
+    }
+    // Lists in the raw order (count per cell, concatenated index/value)
+    void Upload(const int32_t* count, const int32_t* index, const double* value)
+    {
+        ensure_context();
+        check(mpf_upload(ctx_, count, index, value), ctx_, "PhaseField::Read");
+    }
+    // PhaseField::Write(FileName) (src/PhaseField.cpp:2340-2388)
+    bool Write(const std::string& FileName) const
+    {
+        if (!ctx_) return false;
+        std::vector<int32_t> count, index;
+        std::vector<double> value;
+        Download(count, index, value);
+        std::ofstream out(FileName, std::ios::binary);
+        if (!out) return false;
+        int n[3] = {Grid.Nx ? Grid.Nx : 1, Grid.Ny ? Grid.Ny : 1, Grid.Nz ? Grid.Nz : 1};
+        out.write(reinterpret_cast<const char*>(n), sizeof(n));
+        size_t off = 0;
+        for (size_t c = 0; c < count.size(); ++c) {
+            uint64_t k = static_cast<uint64_t>(count[c]);
+            out.write(reinterpret_cast<const char*>(&k), sizeof(k));
+            for (uint64_t q = 0; q < k; ++q, ++off) {
+                uint64_t idx = static_cast<uint64_t>(index[off]);
+                out.write(reinterpret_cast<const char*>(&idx), sizeof(idx));
+                out.write(reinterpret_cast<const char*>(&value[off]), sizeof(double));
+            }
+        }
+        return static_cast<bool>(out);
+    }
+    void Download(std::vector<int32_t>& count, std::vector<int32_t>& index, std::vector<double>& value) const
+    {
+        int64_t total = 0;
+        check(mpf_total_entries(ctx_, &total), ctx_, "PhaseField::Download");
+        count.resize(static_cast<size_t>(Grid.TotalNumberOfCells()));
+        index.resize(static_cast<size_t>(total));
+        value.resize(static_cast<size_t>(total));
+        check(mpf_download(ctx_, count.data(), index.data(), value.data(), nullptr), ctx_, "PhaseField::Download");
+    }
+
+    // PhaseField::FinalizeInitialization (src/PhaseField.cpp:1068-1077)
+    void FinalizeInitialization(const BoundaryConditions& BC)
+    {
+        check_bc(BC);
+        ensure_context();
+        check(mpf_finalize_initialization(ctx_), ctx_, "PhaseField::FinalizeInitialization");
+        stage_ = 0;
+    }
+    // PhaseField::NormalizeIncrements (src/PhaseField.cpp:1334-1349): third call of a step
+    void NormalizeIncrements(const BoundaryConditions& BC, const double dt)
+    {
+        check_bc(BC);
+        if (stage_ != 2) throw std::logic_error("NormalizeIncrements before IP.Set and CalculatePhaseFieldIncrements");
+        dt_ = dt;
+        stage_ = 3;
+    }
+    // PhaseField::MergeIncrements (src/PhaseField.cpp:1248-1266) incl. Finalize: runs the step
+    void MergeIncrements(const BoundaryConditions& BC, const double dt, const bool finalize = true,
+                         const bool clear = true)
+    {
+        check_bc(BC);
+        if (!finalize || !clear) throw std::logic_error("MergeIncrements without finalize/clear is not supported");
+        if (stage_ != 3 || dt != dt_) throw std::logic_error("MergeIncrements must follow NormalizeIncrements with the same dt");
+        check(mpf_step(ctx_, dt, 1), ctx_, "PhaseField::MergeIncrements");
+        stage_ = 0;
+    }
+    // Runs n steps of the recorded sequence at once (one synchronisation)
+    void Advance(double dt, int nsteps)
+    {
+        check(mpf_step(ctx_, dt, nsteps), ctx_, "PhaseField::Advance");
+    }
+    // GrainsProperties Volume / VolumeRatio
+    void GrainVolumes(std::vector<double>& volume, std::vector<double>& ratio) const
+    {
+        volume.resize(grain_phase_.size());
+        ratio.resize(grain_phase_.size());
+        check(mpf_get_grains(ctx_, volume.data(), ratio.data()), ctx_, "PhaseField::GrainVolumes");
+    }
+
+    GridParameters Grid;
+    size_t Nphases;
+    bool ConsiderNucleusVolume = true;
+    double NucleusVolumeFactor = 1.0;
+
+ private:
+    friend class InterfaceProperties;
+    friend class DoubleObstacle;
+    friend class DrivingForce;
+    friend class UserDrivingForce;
+
+    void check_bc(const BoundaryConditions& BC)
+    {
+        for (int i = 0; i < 6; ++i) bc_[i] = BC.bc[i];
+    }
+    void ensure_context();
+
+    Settings& settings_;
+    std::string file_;
+    int precision_;
+    int slots_;
+    int32_t bc_[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<int32_t> grain_phase_;
+    std::vector<double> grain_rot_;
+    bool grains_dirty_ = true;
+    mpf_ctx* ctx_ = nullptr;
+    int stage_ = 0;      // 0 idle, 1 IP.Set done, 2 increments done, 3 normalized
+    double dt_ = 0.0;
+    bool driving_force_ = false;
+};
+
+// ------------------------------------------------------------------------------------------
+// InterfaceProperties (include/InterfaceProperties.h): phase-pair energy/mobility models.
+class InterfaceProperties {
+ public:
+    InterfaceProperties(Settings& s, const std::string& file) : Nphases(s.Nphases), dx_(s.Grid.dx), active_(s.Grid.Active())
+    {
+        InputFile in(file);
+        const size_t np2 = Nphases * Nphases;
+        Sigma.assign(np2, 0.0);
+        Mu.assign(np2, 0.0);
+        EnergyModel.assign(np2, MPF_ENERGY_ISO);
+        Epsilon.assign(np2, 0.0);
+        for (size_t a = 0; a < Nphases; ++a)
+            for (size_t b = a; b < Nphases; ++b) {
+                const std::string c = "_" + std::to_string(a) + "_" + std::to_string(b);
+                const std::string em = in.get("InterfaceProperties", "EnergyModel" + c, "ISO");
+                const std::string mm = in.get("InterfaceProperties", "MobilityModel" + c, "ISO");
+                int model = MPF_ENERGY_ISO;
+                if (em == "CUBIC") model = MPF_ENERGY_CUBIC;
+                else if (em != "ISO") throw std::runtime_error("EnergyModel" + c + " " + em + " is not implemented");
+                if (mm != "ISO") throw std::runtime_error("MobilityModel" + c + " " + mm + " is not implemented");
+                const double sig = in.getd("InterfaceProperties", "Sigma" + c, 0.0, true);
+                const double mu = in.getd("InterfaceProperties", "Mu" + c, 0.0, true);
+                const double eps = model == MPF_ENERGY_CUBIC ? in.getd("InterfaceProperties", "EpsilonE" + c, 0.0, true) : 0.0;
+                for (auto k : {a * Nphases + b, b * Nphases + a}) {
+                    Sigma[k] = sig; Mu[k] = mu; EnergyModel[k] = model; Epsilon[k] = eps;
+                }
+            }
+        const std::string tj = in.get("InterfaceProperties", "TripleJunctionModel", "MODEL");
+        if (tj == "MODEL") { TripleJunctionModel = MPF_TJ_MODEL; TripleJunctionFactor = in.getd("InterfaceProperties", "TripleJunctionFactor", 0.0); }
+        else if (tj == "VALUE") { TripleJunctionModel = MPF_TJ_VALUE; TripleJunctionEnergy = in.getd("InterfaceProperties", "TripleJunctionEnergy", 0.0, true); }
+        else if (tj == "NONE") TripleJunctionModel = MPF_TJ_NONE;
+        else throw std::runtime_error("unknown TripleJunctionModel " + tj);
+        RegularizationFactor = in.getd("InterfaceProperties", "RegularizationFactor", 1.0);
+    }
+    // InterfaceProperties::Set (src/InterfaceProperties.cpp:557-576): first call of a step
+    void Set(PhaseField& Phase, const BoundaryConditions& BC)
+    {
+        Phase.check_bc(BC);
+        Phase.ensure_context();
+        Phase.stage_ = 1;
+    }
+    // InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556)
+    double ReportMaximumTimeStep() const
+    {
+        double me = 0.0, mm = 0.0;
+        for (size_t k = 0; k < Sigma.size(); ++k) {
+            const double maxe = EnergyModel[k] == MPF_ENERGY_CUBIC ? Sigma[k] * (1.0 + Epsilon[k] * 1.41) : Sigma[k];
+            me = std::max(me, maxe);
+            mm = std::max(mm, Mu[k]);
+        }
+        const double f = active_ == 1 ? 0.50 : (active_ == 2 ? 0.25 : 0.16);
+        return f * dx_ * dx_ / (mm * me * RegularizationFactor);
+    }
+    size_t Nphases;
+    std::vector<double> Sigma, Mu, Epsilon;
+    std::vector<int32_t> EnergyModel;
+    int TripleJunctionModel = MPF_TJ_MODEL;
+    double TripleJunctionFactor = 0.0;
+    double TripleJunctionEnergy = 0.0;
+    double RegularizationFactor = 1.0;
+
+ private:
+    friend class PhaseField;
+    double dx_;
+    int active_;
+};
+
+// ------------------------------------------------------------------------------------------
+// UserDrivingForce / DrivingForce (constant phase-pair driving force, no averaging)
+class UserDrivingForce {
+ public:
+    UserDrivingForce(Settings& s, const std::string& file) : Nphases(s.Nphases)
+    {
+        InputFile in(file);
+        Mode.assign(Nphases * Nphases, 0);
+        Value.assign(Nphases * Nphases, 0.0);
+        for (size_t n = 0; n < Nphases; ++n)
+            for (size_t m = n + 1; m < Nphases; ++m) {
+                const std::string c = "_" + std::to_string(n) + "_" + std::to_string(m);
+                const std::string mode = in.get("UserDrivingForce", "UDF_Mode" + c, "NONE");
+                if (mode == "VALUE") {
+                    const double v = in.getd("UserDrivingForce", "UDF_Value" + c, 0.0, true);
+                    Mode[n * Nphases + m] = Mode[m * Nphases + n] = 1;
+                    Value[n * Nphases + m] = v;
+                    Value[m * Nphases + n] = -v;
+                } else if (mode != "NONE") {
+                    throw std::runtime_error("UDF_Mode" + c + " " + mode + " is not implemented");
+                }
+            }
+    }
+    void SetDrivingForce(PhaseField& Phase, DrivingForce&) { (void)Phase; }
+    size_t Nphases;
+    std::vector<int32_t> Mode;
+    std::vector<double> Value;
+};
+
+class DrivingForce {
+ public:
+    DrivingForce(Settings& s, const std::string& file) : Nphases(s.Nphases)
+    {
+        InputFile in(file);
+        if (in.getb("DrivingForce", "Average", true))
+            throw std::runtime_error("DrivingForce averaging is not implemented: set $Average : No");
+        Limit = in.getd("DrivingForce", "Limit_0_0", 0.95);
+    }
+    void Clear() {}
+    void Average(const PhaseField&, const BoundaryConditions&) {}
+    void MergePhaseFieldIncrements(PhaseField&, InterfaceProperties&) {}
+    size_t Nphases;
+    double Limit = 0.95;
+};
+
+// ------------------------------------------------------------------------------------------
+// DoubleObstacle (include/DoubleObstacle.h)
+class DoubleObstacle {
+ public:
+    DoubleObstacle() = default;
+    DoubleObstacle(Settings&, const std::string&) {}
+    // DoubleObstacle::CalculatePhaseFieldIncrements (src/DoubleObstacle.cpp:461-500)
+    void CalculatePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties&)
+    {
+        if (Phase.stage_ != 1) throw std::logic_error("CalculatePhaseFieldIncrements before IP.Set");
+        Phase.stage_ = 2;
+    }
+    void CalculatePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties& IP, DrivingForce&)
+    {
+        CalculatePhaseFieldIncrements(Phase, IP);
+    }
+};
+
+// Builds the device context from the input file the first time the state is needed.
+inline void PhaseField::ensure_context()
+{
+    if (ctx_ && !grains_dirty_) return;
+    if (!ctx_) {
+        InterfaceProperties ip(settings_, file_);
+        std::unique_ptr<UserDrivingForce> udf;
+        {
+            InputFile in(file_);
+            bool any = false;
+            for (size_t n = 0; n < Nphases; ++n)
+                for (size_t m = n + 1; m < Nphases; ++m)
+                    any = any || in.has("UserDrivingForce", "UDF_Mode_" + std::to_string(n) + "_" + std::to_string(m));
+            if (any) udf.reset(new UserDrivingForce(settings_, file_));
+        }
+        mpf_grid g{};
+        g.nx = Grid.Nx; g.ny = Grid.Ny; g.nz = Grid.Nz;
+        g.dx = Grid.dx; g.iwidth = Grid.iWidth;
+        for (int i = 0; i < 6; ++i) g.bc[i] = bc_[i];
+        g.precision = precision_;
+        g.slots = slots_;
+        mpf_params p{};
+        p.nphases = static_cast<int32_t>(Nphases);
+        p.sigma = ip.Sigma.data();
+        p.mu = ip.Mu.data();
+        p.energy_model = ip.EnergyModel.data();
+        p.epsilon = ip.Epsilon.data();
+        p.tj_model = ip.TripleJunctionModel;
+        p.tj_factor = ip.TripleJunctionFactor;
+        p.tj_energy = ip.TripleJunctionEnergy;
+        p.consider_nucleus_volume = ConsiderNucleusVolume ? 1 : 0;
+        p.nucleus_volume_factor = NucleusVolumeFactor;
+        p.dg_mode = udf ? udf->Mode.data() : nullptr;
+        p.dg_value = udf ? udf->Value.data() : nullptr;
+        p.dg_limit = 0.95;
+        p.regularization_factor = ip.RegularizationFactor;
+        check(mpf_create(&g, &p, &ctx_), nullptr, "mpf_create");
+        driving_force_ = static_cast<bool>(udf);
+    }
+    if (grains_dirty_) {
+        if (grain_phase_.empty()) throw std::logic_error("no grains: call AddGrainInfo first");
+        std::vector<uint8_t> solid(grain_phase_.size(), 1);
+        for (size_t g = 0; g < grain_phase_.size(); ++g)
+            if (static_cast<size_t>(grain_phase_[g]) < settings_.Solid.size()) solid[g] = settings_.Solid[grain_phase_[g]];
+        check(mpf_set_grains(ctx_, static_cast<int32_t>(grain_phase_.size()), grain_phase_.data(), grain_rot_.data(),
+                             solid.data()),
+              ctx_, "PhaseField::AddGrainInfo");
+        grains_dirty_ = false;
+    }
+}
+
+}  // namespace openphase_amd
+#endif

@@ -60,6 +60,15 @@ def build_hip() -> Path:
     return LIB
 
 
+def build_examples() -> None:
+    """The C++ drop-in example (OpenPhase-style classes over the C ABI), linked to libmpf_hip.so."""
+    cxx = shutil.which("g++") or "g++"
+    src = PKG / "examples" / "normal_gg.cpp"
+    out = PKG / "examples" / "normal_gg"
+    _run([cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(src), f"-L{PKG}", "-lmpf_hip",
+          "-Wl,-rpath,$ORIGIN/..", "-o", str(out)])
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
     ref = Path(os.environ.get("OPENPHASE_REFERENCE", "/root/reference"))
@@ -70,9 +79,10 @@ def build_oracle() -> None:
 
 def build_all() -> None:
     build_hip()
+    build_examples()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"hip": build_hip, "oracle": build_oracle, "all": build_all}[what]()
+    {"hip": build_hip, "examples": build_examples, "oracle": build_oracle, "all": build_all}[what]()

@@ -52,6 +52,7 @@ struct mpf_ctx {
     std::vector<double> sigma, mu, eps, maxe, dgval, dgallowed;
     std::vector<int32_t> emodel, dgmode;
     double tj_factor = 0.0, tj_energy = 0.0;
+    bool uniform = false;            // see StepArgs::uniform
     int tj_model = 0;
     int consider_nucleus = 1;
     double nucleus_volume_factor = 1.0, regularization = 1.0;
@@ -90,6 +91,7 @@ struct mpf_ctx {
     unsigned long long* d_warnings = nullptr;
     uint32_t* d_work_list = nullptr;
     unsigned* d_work_count = nullptr;
+    unsigned long long* d_stats = nullptr;
 
     bool have_state = false, have_grains = false, initialized = false;
     bool timing = false;
@@ -136,6 +138,9 @@ mpf::StepArgs step_args(const mpf_ctx* c, double dt, int src)
     a.tj_model = c->tj_model;
     a.tj_factor = c->tj_factor;
     a.tj_energy = c->tj_energy;
+    a.uniform = c->uniform ? 1 : 0;
+    a.sigma0 = c->sigma[0];
+    a.mu0 = c->mu[0];
     a.sigma = c->d_sigma; a.mu = c->d_mu; a.eps = c->d_eps; a.max_energy = c->d_maxe;
     a.emodel = c->d_emodel; a.dg_mode = c->d_dgmode; a.dg_value = c->d_dgval; a.dg_allowed = c->d_dgallowed;
     a.gphase = c->g_phase; a.gvr = c->g_vr; a.grot = c->g_rot; a.gsolid = c->g_solid;
@@ -162,6 +167,7 @@ mpf::GrainArgs grain_args(const mpf_ctx* c)
     g.exist = c->g_exist;
     g.stage = c->g_stage;
     g.work_count = c->d_work_count;
+    g.stats = c->d_stats;
     return g;
 }
 
@@ -234,6 +240,9 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     c->tj_model = params->tj_model;
     c->tj_factor = params->tj_factor;
     c->tj_energy = params->tj_energy;
+    c->uniform = !c->grad && ((c->tj_model == MPF_TJ_MODEL && c->tj_factor == 0.0) || c->tj_model == MPF_TJ_NONE);
+    for (size_t i = 0; i < np2; ++i)
+        if (c->sigma[i] != c->sigma[0] || c->mu[i] != c->mu[0]) c->uniform = false;
     c->consider_nucleus = params->consider_nucleus_volume;
     c->nucleus_volume_factor = params->nucleus_volume_factor;
     c->regularization = params->regularization_factor;
@@ -293,7 +302,8 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         c->hip(dalloc(&c->d_dgval, np2), "tables") || c->hip(dalloc(&c->d_dgallowed, np2), "tables") ||
         c->hip(dalloc(&c->d_status, mpf::kStatusWords), "status") ||
         c->hip(dalloc(&c->d_first_overflow, 1), "status") || c->hip(dalloc(&c->d_warnings, 1), "status") ||
-        c->hip(dalloc(&c->d_work_list, N), "work list") || c->hip(dalloc(&c->d_work_count, 1), "work list"))
+        c->hip(dalloc(&c->d_work_list, 2 * N), "work list") || c->hip(dalloc(&c->d_work_count, 2), "work list") ||
+        c->hip(dalloc(&c->d_stats, 2), "statistics"))
         return bail(MPF_ERR_NOMEM, c->err.c_str());
     hipError_t e = hipSuccess;
     h2d(e, c->d_sigma, c->sigma.data(), np2);
@@ -309,8 +319,10 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     h2d(e, c->d_status, zeros, mpf::kStatusWords);
     h2d(e, c->d_first_overflow, &none, 1);
     h2d(e, c->d_warnings, &zero, 1);
-    const unsigned zero_u = 0u;
-    h2d(e, c->d_work_count, &zero_u, 1);
+    const unsigned zero_u[2] = {0u, 0u};
+    h2d(e, c->d_work_count, zero_u, 2);
+    const unsigned long long zero_s[2] = {0ull, 0ull};
+    h2d(e, c->d_stats, zero_s, 2);
     if ((s = c->hip(e, "init copies"))) return bail(s, c->err.c_str());
     *out = c;
     return MPF_OK;
@@ -446,7 +458,7 @@ mpf_status mpf_step_async(mpf_ctx* c, double dt, int32_t nsteps)
         mpf_status st = record(3 * s);
         if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_step(a, c->fp32, c->grad || c->dg, c->stream)), "step kernel");
         if (!st) st = record(3 * s + 1);
-        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_wide(a, c->fp32, c->grad || c->dg, c->stream)), "deferred-cell kernel");
+        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(a, c->fp32, c->grad || c->dg, c->stream)), "deferred-cell kernels");
         if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_grains(g, c->stream)), "grain kernel");
         if (!st) st = record(3 * s + 2);
         if (st) return st;
@@ -614,6 +626,45 @@ mpf_status mpf_enable_timing(mpf_ctx* c, int32_t on)
     return MPF_OK;
 }
 
+mpf_status mpf_step_stats(mpf_ctx* c, int64_t* deferred_mid, int64_t* deferred_wide)
+{
+    if (!c) return MPF_ERR_ARG;
+    unsigned long long s[2] = {0ull, 0ull};
+    mpf_status st = c->hip(hipMemcpyAsync(s, c->d_stats, sizeof(s), hipMemcpyDeviceToHost, c->stream), "statistics");
+    if (!st) st = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (deferred_mid) *deferred_mid = static_cast<int64_t>(s[0]);
+    if (deferred_wide) *deferred_wide = static_cast<int64_t>(s[1]);
+    return st;
+}
+
+mpf_status mpf_copy_bandwidth(mpf_ctx* c, int64_t bytes, int32_t repeats, double* ms)
+{
+    if (!c || !ms) return MPF_ERR_ARG;
+    if (bytes <= 0 || bytes % 16 || repeats < 1) return c->fail(MPF_ERR_ARG, "bytes must be a positive multiple of 16, repeats >= 1");
+    void* src = nullptr;
+    void* dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    mpf_status s = c->hip(hipMalloc(&src, static_cast<size_t>(bytes)), "hipMalloc copy source");
+    if (!s) s = c->hip(hipMalloc(&dst, static_cast<size_t>(bytes)), "hipMalloc copy target");
+    if (!s) s = c->hip(hipMemsetAsync(src, 1, static_cast<size_t>(bytes), c->stream), "hipMemsetAsync");
+    if (!s) s = c->hip(hipEventCreate(&e0), "hipEventCreate");
+    if (!s) s = c->hip(hipEventCreate(&e1), "hipEventCreate");
+    if (!s) s = c->hip(static_cast<hipError_t>(mpf::launch_copy(src, dst, bytes, c->stream)), "copy kernel");  // warm-up
+    if (!s) s = c->hip(hipEventRecord(e0, c->stream), "hipEventRecord");
+    for (int r = 0; r < repeats && !s; ++r)
+        s = c->hip(static_cast<hipError_t>(mpf::launch_copy(src, dst, bytes, c->stream)), "copy kernel");
+    if (!s) s = c->hip(hipEventRecord(e1, c->stream), "hipEventRecord");
+    if (!s) s = c->hip(hipEventSynchronize(e1), "hipEventSynchronize");
+    float t = 0.f;
+    if (!s) s = c->hip(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+    if (!s) *ms = static_cast<double>(t) / repeats;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(src);
+    dfree(dst);
+    return s;
+}
+
 const char* mpf_error_message(mpf_ctx* c)
 {
     return c ? c->err.c_str() : "null context";
@@ -629,7 +680,7 @@ void mpf_destroy(mpf_ctx* c)
     dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
-    dfree(c->d_work_list); dfree(c->d_work_count);
+    dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

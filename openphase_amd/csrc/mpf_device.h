@@ -8,7 +8,7 @@ namespace mpf {
 constexpr int kMaxSlots = 16;                        // largest per-cell list capacity of the stored state
 constexpr int kBlock = 256;                          // threads per block of the step kernels
 constexpr int kWideBlock = 64;                       // threads per block of the deferred-cell kernel
-constexpr int kWideGrid = 512;                       // blocks of the deferred-cell kernel (grid-stride)
+constexpr int kWideGrid = 4096;                      // blocks of the deferred-cell kernels (grid-stride)
 constexpr int kLimbs = 4;                            // 32-bit limbs of the exact per-grain volume sums
 
 // Device status words (one int each).
@@ -29,6 +29,12 @@ struct StepArgs {
     int32_t tj_model;            // 0 MODEL, 1 VALUE, 2 NONE
     double tj_factor;
     double tj_energy;
+    // 1 when every phase pair has the same ISO energy and mobility and no triple-junction term
+    // can act at VolumeRatio 1 (MODEL with factor 0, or NONE): then the pairwise gamma sums
+    // (sigma_bg - sigma_ag) * (...) are exact zeros and are skipped
+    int32_t uniform;
+    double sigma0;               // the common energy / mobility when uniform
+    double mu0;
     const double* sigma;         // nphases^2 phase-pair interface energy
     const double* mu;            // nphases^2 phase-pair mobility
     const double* eps;           // nphases^2 cubic anisotropy strength
@@ -51,8 +57,8 @@ struct StepArgs {
     int32_t* status;             // kStatusWords
     unsigned long long* first_overflow;
     unsigned long long* warnings;
-    uint32_t* work_list;         // cells deferred to the wide kernel (capacity ncells)
-    unsigned* work_count;
+    uint32_t* work_list;         // deferred cells: [0, ncells) for the mid kernel, [ncells, 2 ncells) for the wide kernel
+    unsigned* work_count;        // [2] lengths of the two lists
 };
 
 struct GrainArgs {
@@ -64,14 +70,16 @@ struct GrainArgs {
     double* vr;
     uint8_t* exist;
     int32_t* stage;              // 0 seed, 1 nucleus, 2 stable
-    unsigned* work_count;        // reset here for the next step
+    unsigned* work_count;        // [2], reset here for the next step
+    unsigned long long* stats;   // [2] cumulative lengths of the two work lists
 };
 
 // Host-side launchers (mpf_kernels.hip); return a hipError_t value.
 // `full`: orientation-dependent energies and/or constant driving force present.
 int launch_step(const StepArgs& a, bool fp32, bool full, void* stream);
-int launch_wide(const StepArgs& a, bool fp32, bool full, void* stream);
+int launch_deferred(const StepArgs& a, bool fp32, bool full, void* stream);
 int launch_finalize_init(const StepArgs& a, bool fp32, void* stream);
 int launch_grains(const GrainArgs& g, void* stream);
+int launch_copy(const void* src, void* dst, int64_t bytes, void* stream);
 
 }  // namespace mpf

@@ -11,12 +11,14 @@
 // include/Containers/NodePF.h:214-255), and writes the next state. Periodic halos are wrapped
 // indices (BoundaryConditions::SetX/Y/Z, include/BoundaryConditions.h:694-712).
 //
-// The common case (merged list of at most 8 entries) runs fully unrolled in registers. Cells whose
-// merged list is longer (junctions of many grains) are deferred to a work list and finished by a
-// second kernel that holds up to 16 entries per cell in private memory.
+// Work per cell is dominated by the merged-list length n (2-4 in grain growth). The merge runs in
+// registers with capacity 8; the pair/limiter/finalize part is compiled separately for n = 2, 3, 4
+// and for n <= 8, so a wavefront only executes the sizes its cells have. Cells whose merged list is
+// longer than 8 are deferred to a work list finished by a second kernel (capacity 16).
 //
-// The per-grain volume sums (PhaseField::CalculateGrainsVolume, src/PhaseField.cpp:812-951) are
-// accumulated exactly as 128-bit fixed point in 32-bit limbs, so they are deterministic.
+// Per-grain volume sums (PhaseField::CalculateGrainsVolume, src/PhaseField.cpp:812-951) are
+// reduced per wavefront in a fixed lane order and added across wavefronts as 128-bit fixed point
+// integers, so they do not depend on scheduling.
 //
 // Everything is built with -ffp-contract=off: the FP64 path performs the reference's operations
 // in the reference's order, so active-phase sets match bit for bit.
@@ -53,18 +55,19 @@ template <typename T> __device__ __forceinline__ T load_val(const void* p, int64
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
-// Exact fixed-point image V = v * 2^116 of a value in [2^-64, 2) as four 32-bit limbs.
+// Fixed-point image V = v * 2^116 of a non-negative v < 2^12 as four 32-bit limbs (exact for
+// v >= 2^-64; smaller parts are truncated).
 __device__ __forceinline__ void value_limbs(double v, unsigned long long l[kLimbs], bool& bad)
 {
     unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(v));
     int e = static_cast<int>((bits >> 52) & 0x7ff);
     l[0] = l[1] = l[2] = l[3] = 0ull;
     if ((bits >> 63) || e == 0x7ff) { bad = true; return; }
-    if (e == 0) return;  // zero or subnormal: below the fixed-point resolution
+    if (e == 0) return;  // zero or subnormal
     unsigned long long mant = (bits & 0xFFFFFFFFFFFFFull) | (1ull << 52);
     int shift = e - 959;  // (e - 1023) - 52 + 116
+    if (shift > 75) { bad = true; return; }  // v >= 2^12
     unsigned long long lo, hi;
-    if (shift > 64) { bad = true; return; }
     if (shift < 0) {
         lo = shift > -64 ? (mant >> (-shift)) : 0ull;
         hi = 0ull;
@@ -73,7 +76,7 @@ __device__ __forceinline__ void value_limbs(double v, unsigned long long l[kLimb
     } else if (shift < 64) {
         lo = mant << shift; hi = mant >> (64 - shift);
     } else {
-        lo = 0ull; hi = mant;
+        lo = 0ull; hi = mant << (shift - 64);
     }
     l[0] = lo & 0xFFFFFFFFull;
     l[1] = lo >> 32;
@@ -81,32 +84,44 @@ __device__ __forceinline__ void value_limbs(double v, unsigned long long l[kLimb
     l[3] = hi >> 32;
 }
 
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-    return v;
-}
-
-// Adds the (grain, value) entries of all lanes of the wavefront to the grain accumulators: one
-// atomic per distinct grain and limb instead of one per entry. Must be reached by whole waves.
+// Adds the (grain, value) entries of all lanes of the wavefront to the grain accumulators with one
+// integer atomic per distinct grain and limb. Must be reached by whole wavefronts.
+// LANE_EXACT = false: the lanes' values are summed in FP64 by a butterfly (fixed lane order);
+// deterministic when the cell-to-lane mapping is (the main kernel).
+// LANE_EXACT = true: every value is converted to fixed point first and the limbs are summed as
+// integers; deterministic for any mapping (the work-list kernels, whose order varies).
+template <bool LANE_EXACT>
 __device__ __forceinline__ void wave_volume_add(bool act, uint32_t g, double v,
                                                 unsigned long long* acc, bool& bad)
 {
-    unsigned long long l[kLimbs];
-    if (act) value_limbs(v, l, bad);
-    else l[0] = l[1] = l[2] = l[3] = 0ull;
     const int lane = static_cast<int>(threadIdx.x) & (warpSize - 1);
+    unsigned long long lv[kLimbs] = {0ull, 0ull, 0ull, 0ull};
+    if (LANE_EXACT && act) value_limbs(v, lv, bad);
     while (true) {
         unsigned long long m = __ballot(act);
         if (m == 0ull) break;
         int leader = __ffsll(static_cast<long long>(m)) - 1;
         uint32_t g0 = static_cast<uint32_t>(__shfl(static_cast<int>(g), leader));
         bool sel = act && (g == g0);
+        unsigned long long l[kLimbs];
+        if (LANE_EXACT) {
 #pragma unroll
-        for (int i = 0; i < kLimbs; ++i) {
-            unsigned long long s = wave_sum(sel ? l[i] : 0ull);
-            if (lane == leader && s) atomicAdd(&acc[static_cast<int64_t>(g0) * kLimbs + i], s);
+            for (int i = 0; i < kLimbs; ++i) {
+                unsigned long long s = sel ? lv[i] : 0ull;
+#pragma unroll
+                for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k);
+                l[i] = s;
+            }
+        } else {
+            double s = sel ? v : 0.0;
+#pragma unroll
+            for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k);
+            if (lane == leader) value_limbs(s, l, bad);
+        }
+        if (lane == leader) {
+#pragma unroll
+            for (int i = 0; i < kLimbs; ++i)
+                if (l[i]) atomicAdd(&acc[static_cast<int64_t>(g0) * kLimbs + i], l[i]);
         }
         act = act && !sel;
     }
@@ -132,98 +147,78 @@ __device__ __forceinline__ TC cubic_energy(const double* R, TC gx, TC gy, TC gz,
     return energy * (TC(1.0) + eps1 * (TC(1.5) - TC(2.5) * (q0 * q0 + q1 * q1 + q2 * q2)));
 }
 
-// finalize clamp/erase/renormalize of the first n entries (NodePF::finalize)
-template <int K, int U>
-__device__ __forceinline__ void finalize_list(int n, double (&phi)[K], bool (&keep)[K], int& ns, bool& empty)
-{
-    double total = 0.0;
-    ns = 0;
-#pragma unroll U
-    for (int t = 0; t < K; ++t) {
-        keep[t] = false;
-        if (t >= n) continue;
-        double v = phi[t];
-        bool erase = false;
-        if (v >= 1.0 - DBL_EPSILON) v = 1.0;
-        if (v <= 0.0 + DBL_EPSILON) { v = 0.0; erase = true; }
-        total += v;
-        phi[t] = v;
-        keep[t] = !erase;
-        ns += erase ? 0 : 1;
-    }
-    empty = ns == 0;
-    if (ns > 1) {
-        const double norm = 1.0 / total;
-#pragma unroll U
-        for (int t = 0; t < K; ++t) if (keep[t]) phi[t] *= norm;
-    } else if (ns == 1) {
-#pragma unroll U
-        for (int t = 0; t < K; ++t) if (keep[t]) phi[t] = 1.0;
-    }
-}
+enum CellResult { kCellBulk = 0, kCellWide = 1, kCellDeferred = 2, kCellOverflow = 3 };
 
-enum CellResult { kCellDone = 0, kCellDeferred = 1, kCellOverflow = 2 };
+// Per-cell list held in registers (K entries): phase-field index, value (FP64 and in the stencil
+// arithmetic), Laplacian and gradient of the neighbour-merged list.
+template <int K, typename TC, bool GRAD> struct CellList {
+    uint32_t eid[K];
+    double phi[K];
+    TC ev[K];
+    TC lap[K];
+    TC gr[GRAD ? K : 1][3];
+    bool keep[K];
+    int n0;  // own entries (flag 2 when > 1)
+    int ne;  // merged entries
+};
 
-// Advances cell cc by one step with a merged-list capacity K. On kCellDone, eid/phi/keep hold the
-// finalized list (in order) and the next state is written; on kCellDeferred (merged list longer
-// than K) nothing is written.
-template <int K, typename TV, typename TC, bool GRAD, bool DG>
-__device__ __forceinline__ int advance_cell(const StepArgs& A, int64_t cc, uint32_t (&eid)[K],
-                                            double (&phi)[K], bool (&keep)[K], bool& empty)
+// Loads the own list and, for wide cells, merges the neighbour lists in stencil order (x outer,
+// z inner) while accumulating Laplacians/gradients. Returns kCellBulk (nothing to do),
+// kCellWide, or kCellDeferred when more than K entries are needed.
+template <int K, typename TV, typename TC, bool GRAD>
+__device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellList<K, TC, GRAD>& L)
 {
-    constexpr int U = K <= kFastSlots ? K : 1;                 // full unroll only in registers
-    constexpr int NPK = K * (K - 1) / 2;
+    constexpr int U = K <= kFastSlots ? K : 1;  // full unroll only in registers
     const int64_t N = A.ncells;
     const int NY = A.ny, NZ = A.nz;
     const int x = static_cast<int>(cc / (static_cast<int64_t>(NY) * NZ));
     const int y = static_cast<int>((cc / NZ) % NY);
     const int z = static_cast<int>(cc % NZ);
     const int n0 = A.cnt_in[cc];
-    empty = false;
+    L.n0 = n0;
+    L.ne = n0;
+#pragma unroll U
+    for (int t = 0; t < K; ++t) {
+        L.eid[t] = 0u; L.phi[t] = 0.0; L.ev[t] = TC(0); L.lap[t] = TC(0); L.keep[t] = false;
+        if (GRAD) { L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][2] = TC(0); }
+    }
     if (n0 > K) return kCellDeferred;
+#pragma unroll U
+    for (int t = 0; t < K; ++t)
+        if (t < n0) {
+            const TV v = load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc);
+            L.eid[t] = A.id_in[static_cast<int64_t>(t) * N + cc];
+            L.phi[t] = static_cast<double>(v);
+            L.ev[t] = static_cast<TC>(v);
+            L.keep[t] = true;
+        }
+
+    // neighbour coordinates (periodic wrap)
+    int xs[3], ys[3], zs[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        xs[d] = wrap1(x + d - 1, A.nx);
+        ys[d] = wrap1(y + d - 1, NY);
+        zs[d] = wrap1(z + d - 1, NZ);
+    }
 
     // wide_interface(): flag 2 (more than one entry) or a flag-2 cell in the 3^d box
     bool wide = n0 > 1;
     if (!wide) {
-        for (int ii = -A.dnx; ii <= A.dnx && !wide; ++ii) {
-            const int xx = wrap1(x + ii, A.nx);
-            for (int jj = -A.dny; jj <= A.dny && !wide; ++jj) {
-                const int yy = wrap1(y + jj, NY);
+        for (int ii = -A.dnx; ii <= A.dnx; ++ii)
+            for (int jj = -A.dny; jj <= A.dny; ++jj)
                 for (int kk = -A.dnz; kk <= A.dnz; ++kk) {
-                    const int zz = wrap1(z + kk, NZ);
-                    const int64_t nb = (static_cast<int64_t>(xx) * NY + yy) * NZ + zz;
-                    if (A.cnt_in[nb] > 1) { wide = true; break; }
+                    const int64_t nb = (static_cast<int64_t>(xs[ii + 1]) * NY + ys[jj + 1]) * NZ + zs[kk + 1];
+                    wide = wide || A.cnt_in[nb] > 1;
                 }
-            }
-        }
     }
+    if (!wide) return kCellBulk;
 
-    TC ev[K];
-    TC lap[K];
-    TC gr[GRAD ? K : 1][3];
-#pragma unroll U
-    for (int t = 0; t < K; ++t) {
-        eid[t] = 0u; phi[t] = 0.0; ev[t] = TC(0); lap[t] = TC(0); keep[t] = false;
-        if (GRAD) { gr[GRAD ? t : 0][0] = gr[GRAD ? t : 0][1] = gr[GRAD ? t : 0][2] = TC(0); }
-        if (t < n0) {
-            const TV v = load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc);
-            eid[t] = A.id_in[static_cast<int64_t>(t) * N + cc];
-            phi[t] = static_cast<double>(v);
-            ev[t] = static_cast<TC>(v);
-            keep[t] = true;
-        }
-    }
-    if (!wide) return kCellDone;  // bulk cell: unchanged
-
-    // ---- derivatives + neighbour-list merge, stencil order x outer, z inner
     int ne = n0;
-    for (int ii = -A.dnx; ii <= A.dnx; ++ii) {
-        const int xx = wrap1(x + ii, A.nx);
-        for (int jj = -A.dny; jj <= A.dny; ++jj) {
-            const int yy = wrap1(y + jj, NY);
+    for (int ii = -A.dnx; ii <= A.dnx; ++ii)
+        for (int jj = -A.dny; jj <= A.dny; ++jj)
             for (int kk = -A.dnz; kk <= A.dnz; ++kk) {
-                const int zz = wrap1(z + kk, NZ);
-                const int64_t nb = (static_cast<int64_t>(xx) * NY + yy) * NZ + zz;
+                const int64_t nb = (static_cast<int64_t>(xs[ii + 1]) * NY + ys[jj + 1]) * NZ + zs[kk + 1];
                 const int nnz = (ii != 0) + (jj != 0) + (kk != 0);
                 const TC w = static_cast<TC>(A.wl[nnz]);
                 const TC gw = static_cast<TC>(A.wg[nnz]);
@@ -236,150 +231,192 @@ __device__ __forceinline__ int advance_cell(const StepArgs& A, int64_t cc, uint3
                     if (gv == TC(0)) continue;
                     const uint32_t gid = A.id_in[static_cast<int64_t>(q) * N + nb];
                     const TC l = w * gv;
+                    const TC lx = GRAD && nnz ? gwx * gv : TC(0);
+                    const TC ly = GRAD && nnz ? gwy * gv : TC(0);
+                    const TC lz = GRAD && nnz ? gwz * gv : TC(0);
                     bool found = false;
+                    // branch-free: adding +0 leaves a sum unchanged (the sums are never -0)
 #pragma unroll U
-                    for (int t = 0; t < K; ++t)
-                        if (t < ne && eid[t] == gid) {
-                            lap[t] += l;
-                            if (GRAD && nnz) {
-                                gr[GRAD ? t : 0][0] = gr[GRAD ? t : 0][0] + gwx * gv;
-                                gr[GRAD ? t : 0][1] = gr[GRAD ? t : 0][1] + gwy * gv;
-                                gr[GRAD ? t : 0][2] = gr[GRAD ? t : 0][2] + gwz * gv;
-                            }
-                            found = true;
+                    for (int t = 0; t < K; ++t) {
+                        const bool mt = t < ne && L.eid[t] == gid;
+                        L.lap[t] += mt ? l : TC(0);
+                        if (GRAD && nnz) {
+                            L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][0] + (mt ? lx : TC(0));
+                            L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][1] + (mt ? ly : TC(0));
+                            L.gr[GRAD ? t : 0][2] = L.gr[GRAD ? t : 0][2] + (mt ? lz : TC(0));
                         }
+                        found = found || mt;
+                    }
                     if (!found) {
                         if (ne >= K) return kCellDeferred;
 #pragma unroll U
                         for (int t = 0; t < K; ++t)
                             if (t == ne) {
-                                eid[t] = gid;
-                                phi[t] = 0.0;
-                                ev[t] = TC(0);
-                                lap[t] = l;
+                                L.eid[t] = gid;
+                                L.lap[t] = l;
                                 if (GRAD) {
-                                    gr[GRAD ? t : 0][0] = nnz ? gwx * gv : TC(0);
-                                    gr[GRAD ? t : 0][1] = nnz ? gwy * gv : TC(0);
-                                    gr[GRAD ? t : 0][2] = nnz ? gwz * gv : TC(0);
+                                    L.gr[GRAD ? t : 0][0] = lx;
+                                    L.gr[GRAD ? t : 0][1] = ly;
+                                    L.gr[GRAD ? t : 0][2] = lz;
                                 }
                             }
                         ++ne;
                     }
                 }
             }
-        }
-    }
+    L.ne = ne;
+    return kCellWide;
+}
+
+// IP.Set + increments + normalize + merge + finalize of a wide cell whose merged list has C
+// entries (EXACT) or at most C entries (ne at run time). Only entries < C are touched.
+template <int C, bool EXACT, int K, typename TC, bool GRAD, bool DG>
+__device__ __forceinline__ void compute_cell(const StepArgs& A, CellList<K, TC, GRAD>& L, bool& empty)
+{
+    constexpr int U = C <= kFastSlots ? C : 1;
+    constexpr int NPC = C * (C - 1) / 2;
+    const int ne = EXACT ? C : L.ne;
+    const int n0 = L.n0;
+    const int NP = A.nphases;
 
     // ---- per-entry grain data
-    int ph[K];
-    double vr[K];
+    int ph[C];
+    double vr[C];
 #pragma unroll U
-    for (int t = 0; t < K; ++t) {
+    for (int t = 0; t < C; ++t) {
         ph[t] = 0; vr[t] = 0.0;
-        if (t < ne) { ph[t] = A.gphase[eid[t]]; vr[t] = A.gvr[eid[t]]; }
+        if (EXACT || t < ne) { ph[t] = A.gphase[L.eid[t]]; vr[t] = A.gvr[L.eid[t]]; }
     }
     double localn = 0.0;
     if (A.consider_nucleus) {
 #pragma unroll U
-        for (int t = 0; t < K; ++t) if (t < ne) localn += vr[t];
+        for (int t = 0; t < C; ++t) if (EXACT || t < ne) localn += vr[t];
     } else {
         localn = static_cast<double>(ne);
     }
     const double norm_1 = 1.0 / localn;
-    const int NP = A.nphases;
-
-    // ---- interface energies of the pairs (ISO: phase-pair table; CUBIC: orientation dependent)
-    TC sig[GRAD ? NPK : 1];
-    if (GRAD) {
+    bool uni = A.uniform != 0;
 #pragma unroll U
-        for (int a = 0; a < K; ++a)
-#pragma unroll U
-        for (int b = 0; b < K; ++b) {
-            if (b <= a || b >= ne) continue;
-            const int k = ph[a] * NP + ph[b];
-            TC en = TC(0);
-            if (A.emodel[k] == 0) {
-                en = static_cast<TC>(A.sigma[k]);
-            } else {
-                const TC E = static_cast<TC>(A.sigma[k]), e1 = static_cast<TC>(A.eps[k]);
-                TC wa = TC(0), wb = TC(0);
-                if (A.gsolid[eid[a]]) {
-                    wa = fabs(ev[a] * (TC(1.0) - ev[a]));
-                    if (wa > TC(DBL_EPSILON))
-                        en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(eid[a]), gr[GRAD ? a : 0][0],
-                                               gr[GRAD ? a : 0][1], gr[GRAD ? a : 0][2], E, e1) * wa;
-                    else
-                        wa = TC(0);
-                }
-                if (A.gsolid[eid[b]]) {
-                    wb = fabs(ev[b] * (TC(1.0) - ev[b]));
-                    if (wb > TC(DBL_EPSILON))
-                        en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(eid[b]), gr[GRAD ? b : 0][0],
-                                               gr[GRAD ? b : 0][1], gr[GRAD ? b : 0][2], E, e1) * wb;
-                    else
-                        wb = TC(0);
-                }
-                if (wa + wb > TC(DBL_EPSILON)) en /= wa + wb;
-                else en = static_cast<TC>(A.max_energy[k]);
-            }
-            sig[GRAD ? pair_index<K>(a, b) : 0] = en;
-        }
-    }
-    auto sigma_of = [&](int a, int b) -> TC {
-        if (GRAD) return a < b ? sig[GRAD ? pair_index<K>(a, b) : 0] : sig[GRAD ? pair_index<K>(b, a) : 0];
-        return static_cast<TC>(A.sigma[ph[a] * NP + ph[b]]);
-    };
+    for (int t = 0; t < C; ++t) uni = uni && (!(EXACT || t < ne) || vr[t] == 1.0);
 
     // ---- double-obstacle pair increments
     const TC P = static_cast<TC>(A.prefactor);
-    double psi[NPK];
+    double psi[NPC];
 #pragma unroll U
-    for (int p = 0; p < NPK; ++p) psi[p] = 0.0;
+    for (int p = 0; p < NPC; ++p) psi[p] = 0.0;
+    if (uni) {
+        // equal energies and VolumeRatio 1: scale = 1, cf = 0.5*(1+1)*P = P and every gamma term
+        // is (sigma - sigma) * (...) = 0, so d = sigma*(X_a - X_b), X = L + P*phi
+        const TC s0 = static_cast<TC>(A.sigma0);
+        const TC mf = static_cast<TC>(A.mu0) * static_cast<TC>(norm_1);
+        TC X[C];
 #pragma unroll U
-    for (int a = 0; a < K; ++a)
+        for (int t = 0; t < C; ++t) X[t] = L.lap[t] + P * L.ev[t];
 #pragma unroll U
-    for (int b = 0; b < K; ++b) {
-        if (b <= a || b >= ne) continue;
-        const double scaled = sqrt(vr[a] * vr[b]);
-        const TC scale = static_cast<TC>(scaled);
-        const TC cf = TC(0.5) * (TC(1.0) + scale) * P;
-        TC d = sigma_of(a, b) * ((lap[a] + cf * ev[a]) - (lap[b] + cf * ev[b]));
-        if (ne > 2) {
+        for (int a = 0; a < C; ++a)
 #pragma unroll U
-            for (int g = 0; g < K; ++g) {
-                if (g == a || g == b || g >= ne) continue;
-                d += (sigma_of(b, g) - sigma_of(a, g)) * (lap[g] + cf * ev[g]);
-                if (A.tj_model == 0) {
-                    if (A.tj_factor != 0.0 || scaled < 1.0) {
-                        const TC e3 = static_cast<TC>(A.sigma[ph[a] * NP + ph[b]]) +
-                                      static_cast<TC>(A.sigma[ph[b] * NP + ph[g]]) +
-                                      static_cast<TC>(A.sigma[ph[a] * NP + ph[g]]);
-                        d += static_cast<TC>(smax(A.tj_factor, (1.0 - scaled))) * P * e3 *
-                             (ev[a] * ev[g] - ev[b] * ev[g]);
+        for (int b = 0; b < C; ++b) {
+            if (b <= a || !(EXACT || b < ne)) continue;
+            TC d = s0 * (X[a] - X[b]);
+            d *= mf;
+            if (DG && n0 > 1 && A.dg_mode[ph[a] * NP + ph[b]] == 1) {
+                const TC raw = static_cast<TC>(A.dg_value[ph[a] * NP + ph[b]]);
+                const TC weight = sqrt(L.ev[a] * L.ev[b]);
+                const TC allowed = static_cast<TC>(A.dg_allowed[ph[a] * NP + ph[b]]);
+                const TC dgl = allowed * tanh(raw / allowed);
+                d += dgl * static_cast<TC>(A.mu0) * weight * static_cast<TC>(A.dg_prefactor);
+            }
+            psi[pair_index<C>(a, b)] = static_cast<double>(d);
+        }
+    } else {
+        // interface energies of the pairs (ISO: phase-pair table; CUBIC: orientation dependent)
+        TC sig[GRAD ? NPC : 1];
+        if (GRAD) {
+#pragma unroll U
+            for (int a = 0; a < C; ++a)
+#pragma unroll U
+            for (int b = 0; b < C; ++b) {
+                if (b <= a || !(EXACT || b < ne)) continue;
+                const int k = ph[a] * NP + ph[b];
+                TC en = TC(0);
+                if (A.emodel[k] == 0) {
+                    en = static_cast<TC>(A.sigma[k]);
+                } else {
+                    const TC E = static_cast<TC>(A.sigma[k]), e1 = static_cast<TC>(A.eps[k]);
+                    TC wa = TC(0), wb = TC(0);
+                    if (A.gsolid[L.eid[a]]) {
+                        wa = fabs(L.ev[a] * (TC(1.0) - L.ev[a]));
+                        if (wa > TC(DBL_EPSILON))
+                            en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(L.eid[a]), L.gr[GRAD ? a : 0][0],
+                                                   L.gr[GRAD ? a : 0][1], L.gr[GRAD ? a : 0][2], E, e1) * wa;
+                        else
+                            wa = TC(0);
                     }
-                } else if (A.tj_model == 1) {
-                    d += static_cast<TC>(A.tj_energy) * P * (ev[a] * ev[g] - ev[b] * ev[g]);
+                    if (A.gsolid[L.eid[b]]) {
+                        wb = fabs(L.ev[b] * (TC(1.0) - L.ev[b]));
+                        if (wb > TC(DBL_EPSILON))
+                            en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(L.eid[b]), L.gr[GRAD ? b : 0][0],
+                                                   L.gr[GRAD ? b : 0][1], L.gr[GRAD ? b : 0][2], E, e1) * wb;
+                        else
+                            wb = TC(0);
+                    }
+                    if (wa + wb > TC(DBL_EPSILON)) en /= wa + wb;
+                    else en = static_cast<TC>(A.max_energy[k]);
                 }
+                sig[GRAD ? pair_index<C>(a, b) : 0] = en;
             }
         }
-        const TC mob = static_cast<TC>(A.mu[ph[a] * NP + ph[b]]);
-        d *= mob * static_cast<TC>(norm_1) * scale;
-        // constant driving force on interface cells (flag 2 = more than one own entry)
-        if (DG && n0 > 1 && A.dg_mode[ph[a] * NP + ph[b]] == 1) {
-            const TC raw = static_cast<TC>(A.dg_value[ph[a] * NP + ph[b]]);
-            const TC weight = sqrt(ev[a] * ev[b]);
-            const TC allowed = static_cast<TC>(A.dg_allowed[ph[a] * NP + ph[b]]);
-            const TC dgl = allowed * tanh(raw / allowed);
-            d += dgl * mob * weight * static_cast<TC>(A.dg_prefactor);
+        auto sigma_of = [&](int a, int b) -> TC {
+            if (GRAD) return a < b ? sig[GRAD ? pair_index<C>(a, b) : 0] : sig[GRAD ? pair_index<C>(b, a) : 0];
+            return static_cast<TC>(A.sigma[ph[a] * NP + ph[b]]);
+        };
+#pragma unroll U
+        for (int a = 0; a < C; ++a)
+#pragma unroll U
+        for (int b = 0; b < C; ++b) {
+            if (b <= a || !(EXACT || b < ne)) continue;
+            const double scaled = sqrt(vr[a] * vr[b]);
+            const TC scale = static_cast<TC>(scaled);
+            const TC cf = TC(0.5) * (TC(1.0) + scale) * P;
+            TC d = sigma_of(a, b) * ((L.lap[a] + cf * L.ev[a]) - (L.lap[b] + cf * L.ev[b]));
+            if (ne > 2) {
+#pragma unroll U
+                for (int g = 0; g < C; ++g) {
+                    if (g == a || g == b || !(EXACT || g < ne)) continue;
+                    d += (sigma_of(b, g) - sigma_of(a, g)) * (L.lap[g] + cf * L.ev[g]);
+                    if (A.tj_model == 0) {
+                        if (A.tj_factor != 0.0 || scaled < 1.0) {
+                            const TC e3 = static_cast<TC>(A.sigma[ph[a] * NP + ph[b]]) +
+                                          static_cast<TC>(A.sigma[ph[b] * NP + ph[g]]) +
+                                          static_cast<TC>(A.sigma[ph[a] * NP + ph[g]]);
+                            d += static_cast<TC>(smax(A.tj_factor, (1.0 - scaled))) * P * e3 *
+                                 (L.ev[a] * L.ev[g] - L.ev[b] * L.ev[g]);
+                        }
+                    } else if (A.tj_model == 1) {
+                        d += static_cast<TC>(A.tj_energy) * P * (L.ev[a] * L.ev[g] - L.ev[b] * L.ev[g]);
+                    }
+                }
+            }
+            const TC mob = static_cast<TC>(A.mu[ph[a] * NP + ph[b]]);
+            d *= mob * static_cast<TC>(norm_1) * scale;
+            // constant driving force on interface cells (flag 2 = more than one own entry)
+            if (DG && n0 > 1 && A.dg_mode[ph[a] * NP + ph[b]] == 1) {
+                const TC raw = static_cast<TC>(A.dg_value[ph[a] * NP + ph[b]]);
+                const TC weight = sqrt(L.ev[a] * L.ev[b]);
+                const TC allowed = static_cast<TC>(A.dg_allowed[ph[a] * NP + ph[b]]);
+                const TC dgl = allowed * tanh(raw / allowed);
+                d += dgl * mob * weight * static_cast<TC>(A.dg_prefactor);
+            }
+            psi[pair_index<C>(a, b)] = static_cast<double>(d);
         }
-        psi[pair_index<K>(a, b)] = static_cast<double>(d);
     }
 
     // ---- normalization / limiter (FP64); alive[p]: pair still in the increment list
+    double (&phi)[K] = L.phi;
     const double dt = A.dt;
-    bool alive[NPK];
+    bool alive[NPC];
 #pragma unroll U
-    for (int p = 0; p < NPK; ++p) alive[p] = false;
+    for (int p = 0; p < NPC; ++p) alive[p] = false;
     if (ne == 2) {
         const double va = psi[0] * dt;
         const double old = phi[0];
@@ -397,29 +434,29 @@ __device__ __forceinline__ int advance_cell(const StepArgs& A, int64_t cc, uint3
     } else if (ne > 2) {
         // zero the pairs that would push a field at 0 below or at 1 above
 #pragma unroll U
-        for (int a = 0; a < K; ++a) {
-            if (a >= ne) continue;
+        for (int a = 0; a < C; ++a) {
+            if (!(EXACT || a < ne)) continue;
             double dpsi = 0.0;
 #pragma unroll U
-            for (int b = 0; b < K; ++b) {
-                if (b == a || b >= ne) continue;
-                dpsi += a < b ? psi[pair_index<K>(a, b)] : psi[pair_index<K>(b, a)] * (-1.0);
+            for (int b = 0; b < C; ++b) {
+                if (b == a || !(EXACT || b < ne)) continue;
+                dpsi += a < b ? psi[pair_index<C>(a, b)] : psi[pair_index<C>(b, a)] * (-1.0);
             }
             if ((phi[a] == 0.0 && dpsi < 0.0) || (phi[a] == 1.0 && dpsi > 0.0)) {
 #pragma unroll U
-                for (int b = 0; b < K; ++b) {
-                    if (b == a || b >= ne) continue;
-                    psi[a < b ? pair_index<K>(a, b) : pair_index<K>(b, a)] = 0.0;
+                for (int b = 0; b < C; ++b) {
+                    if (b == a || !(EXACT || b < ne)) continue;
+                    psi[a < b ? pair_index<C>(a, b) : pair_index<C>(b, a)] = 0.0;
                 }
             }
         }
         bool any = false;
 #pragma unroll U
-        for (int a = 0; a < K; ++a)
+        for (int a = 0; a < C; ++a)
 #pragma unroll U
-        for (int b = 0; b < K; ++b) {
-            if (b <= a || b >= ne) continue;
-            const int p = pair_index<K>(a, b);
+        for (int b = 0; b < C; ++b) {
+            if (b <= a || !(EXACT || b < ne)) continue;
+            const int p = pair_index<C>(a, b);
             alive[p] = psi[p] != 0.0;
             any = any || alive[p];
         }
@@ -430,34 +467,34 @@ __device__ __forceinline__ int advance_cell(const StepArgs& A, int64_t cc, uint3
             while (needed) {
                 ++iter;
                 needed = false;
-                double pos[K], neg[K], lim1[K], lim2[K];
+                double pos[C], neg[C], lim1[C], lim2[C];
 #pragma unroll U
-                for (int t = 0; t < K; ++t) { pos[t] = 0.0; neg[t] = 0.0; }
+                for (int t = 0; t < C; ++t) { pos[t] = 0.0; neg[t] = 0.0; }
 #pragma unroll U
-                for (int a = 0; a < K; ++a)
+                for (int a = 0; a < C; ++a)
 #pragma unroll U
-                for (int b = 0; b < K; ++b) {
-                    if (b <= a || b >= ne) continue;
-                    const int p = pair_index<K>(a, b);
+                for (int b = 0; b < C; ++b) {
+                    if (b <= a || !(EXACT || b < ne)) continue;
+                    const int p = pair_index<C>(a, b);
                     if (!alive[p]) continue;
                     const double v = psi[p];
                     if (v < 0.0) { neg[a] += v; pos[b] += -v; }
                     if (v > 0.0) { pos[a] += v; neg[b] += -v; }
                 }
 #pragma unroll U
-                for (int t = 0; t < K; ++t) {
+                for (int t = 0; t < C; ++t) {
                     lim1[t] = 1.0; lim2[t] = 1.0;
-                    if (t >= ne) continue;
+                    if (!(EXACT || t < ne)) continue;
                     const double npf = phi[t] + (pos[t] + neg[t]) * dt;
                     if (npf < 0.0) lim2[t] = smin(lim2[t], -(phi[t] + pos[t] * dt) / (neg[t] * dt));
                     if (npf > 1.0) lim1[t] = smin(lim1[t], (1.0 - (phi[t] + neg[t] * dt)) / (pos[t] * dt));
                 }
 #pragma unroll U
-                for (int a = 0; a < K; ++a)
+                for (int a = 0; a < C; ++a)
 #pragma unroll U
-                for (int b = 0; b < K; ++b) {
-                    if (b <= a || b >= ne) continue;
-                    const int p = pair_index<K>(a, b);
+                for (int b = 0; b < C; ++b) {
+                    if (b <= a || !(EXACT || b < ne)) continue;
+                    const int p = pair_index<C>(a, b);
                     if (!alive[p]) continue;
                     if (psi[p] < 0.0) {
                         const double t = smin(lim2[a], lim1[b]);
@@ -473,38 +510,38 @@ __device__ __forceinline__ int advance_cell(const StepArgs& A, int64_t cc, uint3
                 if (iter > 24) needed = false;
             }
 #pragma unroll U
-            for (int p = 0; p < NPK; ++p)
+            for (int p = 0; p < NPC; ++p)
                 if (alive[p] && fabs(psi[p] * dt) < DBL_EPSILON) alive[p] = false;
 
             // plausibility check of the limited increments (warning count)
-            double tv[K];
+            double tv[C];
 #pragma unroll U
-            for (int t = 0; t < K; ++t) tv[t] = phi[t];
+            for (int t = 0; t < C; ++t) tv[t] = phi[t];
 #pragma unroll U
-            for (int a = 0; a < K; ++a)
+            for (int a = 0; a < C; ++a)
 #pragma unroll U
-            for (int b = 0; b < K; ++b) {
-                if (b <= a || b >= ne) continue;
-                const int p = pair_index<K>(a, b);
+            for (int b = 0; b < C; ++b) {
+                if (b <= a || !(EXACT || b < ne)) continue;
+                const int p = pair_index<C>(a, b);
                 if (!alive[p] || psi[p] == 0.0) continue;
                 tv[a] += psi[p] * dt;
                 tv[b] += -psi[p] * dt;
             }
             unsigned long long warn = 0;
 #pragma unroll U
-            for (int t = 0; t < K; ++t)
-                if (t < ne && (tv[t] < -FLT_EPSILON || tv[t] > 1.0 + FLT_EPSILON)) ++warn;
+            for (int t = 0; t < C; ++t)
+                if ((EXACT || t < ne) && (tv[t] < -FLT_EPSILON || tv[t] > 1.0 + FLT_EPSILON)) ++warn;
             if (warn) atomicAdd(A.warnings, warn);
         }
     }
 
     // ---- merge
 #pragma unroll U
-    for (int a = 0; a < K; ++a)
+    for (int a = 0; a < C; ++a)
 #pragma unroll U
-    for (int b = 0; b < K; ++b) {
-        if (b <= a || b >= ne) continue;
-        const int p = pair_index<K>(a, b);
+    for (int b = 0; b < C; ++b) {
+        if (b <= a || !(EXACT || b < ne)) continue;
+        const int p = pair_index<C>(a, b);
         if (!alive[p]) continue;
         const double value = 1.0 * psi[p] * dt;
         if (fabs(value) >= DBL_EPSILON) {
@@ -513,31 +550,66 @@ __device__ __forceinline__ int advance_cell(const StepArgs& A, int64_t cc, uint3
         }
     }
 
-    // ---- finalize: clamp, erase, renormalize
+    // ---- finalize: clamp, erase, renormalize (NodePF::finalize)
+    double total = 0.0;
     int ns = 0;
-    finalize_list<K, U>(ne, phi, keep, ns, empty);
-    if (ns > A.slots) return kCellOverflow;
-    return kCellDone;
+#pragma unroll U
+    for (int t = 0; t < C; ++t) {
+        L.keep[t] = false;
+        if (!(EXACT || t < ne)) continue;
+        double v = phi[t];
+        bool erase = false;
+        if (v >= 1.0 - DBL_EPSILON) v = 1.0;
+        if (v <= 0.0 + DBL_EPSILON) { v = 0.0; erase = true; }
+        total += v;
+        phi[t] = v;
+        L.keep[t] = !erase;
+        ns += erase ? 0 : 1;
+    }
+    empty = ns == 0;
+    if (ns > 1) {
+        const double norm = 1.0 / total;
+#pragma unroll U
+        for (int t = 0; t < C; ++t) if (L.keep[t]) phi[t] *= norm;
+    } else if (ns == 1) {
+#pragma unroll U
+        for (int t = 0; t < C; ++t) if (L.keep[t]) phi[t] = 1.0;
+    }
+#pragma unroll
+    for (int t = C; t < K; ++t) L.keep[t] = false;
 }
 
-// Writes the finalized list of cell cc (compacted, order preserved); rounds phi to storage type.
-template <int K, typename TV>
-__device__ __forceinline__ void store_cell(const StepArgs& A, int64_t cc, const uint32_t (&eid)[K],
-                                           double (&phi)[K], const bool (&keep)[K])
+// Writes the finalized list of cell cc (compacted, order preserved); rounds phi to the storage
+// type. Returns false (nothing written) when the list exceeds the stored capacity.
+template <int K, typename TV, typename TC, bool GRAD>
+__device__ __forceinline__ bool store_cell(const StepArgs& A, int64_t cc, CellList<K, TC, GRAD>& L)
 {
     constexpr int U = K <= kFastSlots ? K : 1;
     const int64_t N = A.ncells;
+    int ns = 0;
+#pragma unroll U
+    for (int t = 0; t < K; ++t) ns += L.keep[t] ? 1 : 0;
+    if (ns > A.slots) return false;
     int w = 0;
 #pragma unroll U
     for (int t = 0; t < K; ++t)
-        if (keep[t]) {
-            const TV sv = static_cast<TV>(phi[t]);
-            phi[t] = static_cast<double>(sv);
-            A.id_out[static_cast<int64_t>(w) * N + cc] = static_cast<uint16_t>(eid[t]);
+        if (L.keep[t]) {
+            const TV sv = static_cast<TV>(L.phi[t]);
+            L.phi[t] = static_cast<double>(sv);
+            A.id_out[static_cast<int64_t>(w) * N + cc] = static_cast<uint16_t>(L.eid[t]);
             static_cast<TV*>(A.val_out)[static_cast<int64_t>(w) * N + cc] = sv;
             ++w;
         }
     A.cnt_out[cc] = static_cast<uint8_t>(w);
+    return true;
+}
+
+template <int K, typename TC, bool GRAD>
+__device__ __forceinline__ void report_overflow(const StepArgs& A, int64_t c, CellList<K, TC, GRAD>& L)
+{
+    for (int t = 0; t < K; ++t) L.keep[t] = false;
+    atomicOr(&A.status[kStatusOverflow], 1);
+    atomicMin(A.first_overflow, static_cast<unsigned long long>(c));
 }
 
 template <typename TV, typename TC, bool GRAD, bool DG>
@@ -546,62 +618,102 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs A)
     constexpr int K = kFastSlots;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const bool valid = c < A.ncells;
-    uint32_t eid[K];
-    double phi[K];
-    bool keep[K];
+    CellList<K, TC, GRAD> L;
     bool empty = false, bad_volume = false;
-#pragma unroll
-    for (int t = 0; t < K; ++t) { eid[t] = 0u; phi[t] = 0.0; keep[t] = false; }
+    int r = kCellBulk;
     if (valid) {
-        const int r = advance_cell<K, TV, TC, GRAD, DG>(A, c, eid, phi, keep, empty);
-        if (r == kCellDone) {
-            store_cell<K, TV>(A, c, eid, phi, keep);
-        } else {
-#pragma unroll
-            for (int t = 0; t < K; ++t) keep[t] = false;
-            if (r == kCellDeferred) {
-                const unsigned slot = atomicAdd(A.work_count, 1u);
-                A.work_list[slot] = static_cast<uint32_t>(c);
-            } else {
-                atomicOr(&A.status[kStatusOverflow], 1);
-                atomicMin(A.first_overflow, static_cast<unsigned long long>(c));
+        r = gather_cell<K, TV, TC, GRAD>(A, c, L);
+        if (r == kCellWide) {
+            // merged lists of 2-4 entries here; the rest (few cells) keep this kernel's register
+            // budget low by going to the mid kernel
+            switch (L.ne) {
+            case 2: compute_cell<2, true, K, TC, GRAD, DG>(A, L, empty); break;
+            case 3: compute_cell<3, true, K, TC, GRAD, DG>(A, L, empty); break;
+            case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty); break;
+            default: r = kCellDeferred; break;
             }
         }
+        if (r == kCellDeferred) {
+#pragma unroll
+            for (int t = 0; t < K; ++t) L.keep[t] = false;
+            const unsigned slot = atomicAdd(&A.work_count[0], 1u);
+            A.work_list[slot] = static_cast<uint32_t>(c);
+        } else if (!store_cell<K, TV, TC, GRAD>(A, c, L)) {
+            report_overflow<K, TC, GRAD>(A, c, L);
+        }
         if (empty) atomicOr(&A.status[kStatusEmpty], 1);
+    } else {
+#pragma unroll
+        for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
     }
 #pragma unroll
-    for (int t = 0; t < K; ++t) wave_volume_add(keep[t], eid[t], phi[t], A.vol_acc, bad_volume);
+    for (int t = 0; t < K; ++t) wave_volume_add<false>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
     if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
 }
 
-// Finishes the cells the register kernel deferred (merged list longer than 8 entries).
+// Cells the main kernel deferred: merged lists of 5-8 entries (and any other length) in registers;
+// lists longer than 8 go on to the wide kernel.
 template <typename TV, typename TC, bool GRAD, bool DG>
-__global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs A)
+__global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs A)
 {
-    constexpr int K = kWideSlots;
-    const unsigned n = *A.work_count;
+    constexpr int K = kFastSlots;
+    const unsigned n = A.work_count[0];
     const unsigned stride = gridDim.x * kWideBlock;
     for (unsigned base = blockIdx.x * kWideBlock; base < n; base += stride) {
         const unsigned i = base + threadIdx.x;
         const bool valid = i < n;
-        uint32_t eid[K];
-        double phi[K];
-        bool keep[K];
+        CellList<K, TC, GRAD> L;
         bool empty = false, bad_volume = false;
-        for (int t = 0; t < K; ++t) { eid[t] = 0u; phi[t] = 0.0; keep[t] = false; }
+#pragma unroll
+        for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
         if (valid) {
             const int64_t c = A.work_list[i];
-            const int r = advance_cell<K, TV, TC, GRAD, DG>(A, c, eid, phi, keep, empty);
-            if (r == kCellDone) {
-                store_cell<K, TV>(A, c, eid, phi, keep);
-            } else {
-                for (int t = 0; t < K; ++t) keep[t] = false;
-                atomicOr(&A.status[kStatusOverflow], 1);
-                atomicMin(A.first_overflow, static_cast<unsigned long long>(c));
+            int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
+            if (r == kCellWide) {
+                // 5 and 6 entries in registers; 1, 7, 8 (rare) with the 16-entry path
+                switch (L.ne) {
+                case 5: compute_cell<5, true, K, TC, GRAD, DG>(A, L, empty); break;
+                case 6: compute_cell<6, true, K, TC, GRAD, DG>(A, L, empty); break;
+                default: r = kCellDeferred; break;
+                }
+            }
+            if (r == kCellDeferred) {
+#pragma unroll
+                for (int t = 0; t < K; ++t) L.keep[t] = false;
+                const unsigned slot = atomicAdd(&A.work_count[1], 1u);
+                A.work_list[A.ncells + slot] = static_cast<uint32_t>(c);
+            } else if (!store_cell<K, TV, TC, GRAD>(A, c, L)) {
+                report_overflow<K, TC, GRAD>(A, c, L);
             }
             if (empty) atomicOr(&A.status[kStatusEmpty], 1);
         }
-        for (int t = 0; t < K; ++t) wave_volume_add(keep[t], eid[t], phi[t], A.vol_acc, bad_volume);
+#pragma unroll
+        for (int t = 0; t < K; ++t) wave_volume_add<true>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
+        if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
+    }
+}
+
+// Cells whose merged list is longer than 8 entries (up to 16, held in private memory).
+template <typename TV, typename TC, bool GRAD, bool DG>
+__global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs A)
+{
+    constexpr int K = kWideSlots;
+    const unsigned n = A.work_count[1];
+    const unsigned stride = gridDim.x * kWideBlock;
+    for (unsigned base = blockIdx.x * kWideBlock; base < n; base += stride) {
+        const unsigned i = base + threadIdx.x;
+        const bool valid = i < n;
+        CellList<K, TC, GRAD> L;
+        bool empty = false, bad_volume = false;
+        for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
+        if (valid) {
+            const int64_t c = A.work_list[A.ncells + i];
+            const int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
+            if (r == kCellWide) compute_cell<K, false, K, TC, GRAD, DG>(A, L, empty);
+            if (r == kCellDeferred || !store_cell<K, TV, TC, GRAD>(A, c, L)) report_overflow<K, TC, GRAD>(A, c, L);
+            if (empty) atomicOr(&A.status[kStatusEmpty], 1);
+        }
+        for (int t = 0; t < K; ++t) wave_volume_add<true>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
         if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
     }
 }
@@ -611,37 +723,62 @@ __global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs A)
 template <typename TV>
 __global__ __launch_bounds__(kBlock) void finalize_init_kernel(const StepArgs A)
 {
-    constexpr int K = kFastSlots;
+    constexpr int K = kWideSlots;
     const int64_t N = A.ncells;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const bool valid = c < N;
     const int64_t cc = valid ? c : 0;
     const int n0 = valid ? A.cnt_in[cc] : 0;
-    uint32_t eid[K];
-    double phi[K];
-    bool keep[K];
-    bool bad_volume = false, empty = false;
-#pragma unroll
+    CellList<K, double, false> L;
+    bool bad_volume = false;
     for (int t = 0; t < K; ++t) {
-        eid[t] = 0u; phi[t] = 0.0; keep[t] = false;
+        L.eid[t] = 0u; L.phi[t] = 0.0; L.keep[t] = false;
         if (t < n0) {
-            eid[t] = A.id_in[static_cast<int64_t>(t) * N + cc];
-            phi[t] = static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc));
-            keep[t] = true;
+            L.eid[t] = A.id_in[static_cast<int64_t>(t) * N + cc];
+            L.phi[t] = static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc));
+            L.keep[t] = true;
         }
     }
     if (valid && n0 > 1) {
+        double total = 0.0;
         int ns = 0;
-        finalize_list<K, K>(n0, phi, keep, ns, empty);
-        if (empty) atomicOr(&A.status[kStatusEmpty], 1);
+        for (int t = 0; t < n0; ++t) {
+            double v = L.phi[t];
+            bool erase = false;
+            if (v >= 1.0 - DBL_EPSILON) v = 1.0;
+            if (v <= 0.0 + DBL_EPSILON) { v = 0.0; erase = true; }
+            total += v;
+            L.phi[t] = v;
+            L.keep[t] = !erase;
+            ns += erase ? 0 : 1;
+        }
+        if (ns == 0) atomicOr(&A.status[kStatusEmpty], 1);
+        else if (ns > 1) {
+            const double norm = 1.0 / total;
+            for (int t = 0; t < n0; ++t) if (L.keep[t]) L.phi[t] *= norm;
+        } else {
+            for (int t = 0; t < n0; ++t) if (L.keep[t]) L.phi[t] = 1.0;
+        }
     }
-    if (valid) store_cell<K, TV>(A, cc, eid, phi, keep);
-#pragma unroll
-    for (int t = 0; t < K; ++t) wave_volume_add(keep[t], eid[t], phi[t], A.vol_acc, bad_volume);
+    if (valid) {
+        // TV storage; the list was validated against the capacity on upload
+        const int64_t NN = N;
+        int w = 0;
+        for (int t = 0; t < K; ++t)
+            if (L.keep[t]) {
+                const TV sv = static_cast<TV>(L.phi[t]);
+                L.phi[t] = static_cast<double>(sv);
+                A.id_out[static_cast<int64_t>(w) * NN + cc] = static_cast<uint16_t>(L.eid[t]);
+                static_cast<TV*>(A.val_out)[static_cast<int64_t>(w) * NN + cc] = sv;
+                ++w;
+            }
+        A.cnt_out[cc] = static_cast<uint8_t>(w);
+    }
+    for (int t = 0; t < K; ++t) wave_volume_add<true>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
     if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
 }
 
-// Correctly rounded V * 2^-116 of the exact 4-limb sum.
+// Correctly rounded V * 2^-116 of the 4-limb sum.
 __device__ double limbs_to_double(const unsigned long long l[kLimbs])
 {
     // w = l0 + l1*2^32 + l2*2^64 + l3*2^96 as three 64-bit words
@@ -681,7 +818,12 @@ __device__ double limbs_to_double(const unsigned long long l[kLimbs])
 __global__ void grains_kernel(const GrainArgs G)
 {
     const int g = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g == 0) *G.work_count = 0u;
+    if (g == 0) {
+        G.stats[0] += G.work_count[0];
+        G.stats[1] += G.work_count[1];
+        G.work_count[0] = 0u;
+        G.work_count[1] = 0u;
+    }
     if (g >= G.ngrains) return;
     unsigned long long l[kLimbs];
 #pragma unroll
@@ -708,7 +850,25 @@ __global__ void grains_kernel(const GrainArgs G)
     G.stage[g] = stage;
 }
 
+// Streaming copy (16-byte vectors, grid-stride): the bandwidth reference of the roofline.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+int launch_copy(const void* src, void* dst, int64_t bytes, void* stream)
+{
+    const int64_t n = bytes / 16;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+    return static_cast<int>(hipGetLastError());
+}
 
 // Two variants per precision: the plain isotropic step, and the full one (orientation-dependent
 // energies and constant driving force; with ISO tables or no driving-force pairs it reduces to
@@ -727,16 +887,26 @@ int launch_step(const StepArgs& a, bool fp32, bool full, void* stream)
     return static_cast<int>(hipGetLastError());
 }
 
-int launch_wide(const StepArgs& a, bool fp32, bool full, void* stream)
+int launch_deferred(const StepArgs& a, bool fp32, bool full, void* stream)
 {
     const dim3 grid(kWideGrid);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (fp32) {
-        if (full) hipLaunchKernelGGL((wide_kernel<float, float, true, true>), grid, dim3(kWideBlock), 0, s, a);
-        else      hipLaunchKernelGGL((wide_kernel<float, float, false, false>), grid, dim3(kWideBlock), 0, s, a);
+        if (full) {
+            hipLaunchKernelGGL((mid_kernel<float, float, true, true>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((wide_kernel<float, float, true, true>), grid, dim3(kWideBlock), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((mid_kernel<float, float, false, false>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((wide_kernel<float, float, false, false>), grid, dim3(kWideBlock), 0, s, a);
+        }
     } else {
-        if (full) hipLaunchKernelGGL((wide_kernel<double, double, true, true>), grid, dim3(kWideBlock), 0, s, a);
-        else      hipLaunchKernelGGL((wide_kernel<double, double, false, false>), grid, dim3(kWideBlock), 0, s, a);
+        if (full) {
+            hipLaunchKernelGGL((mid_kernel<double, double, true, true>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((wide_kernel<double, double, true, true>), grid, dim3(kWideBlock), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((mid_kernel<double, double, false, false>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((wide_kernel<double, double, false, false>), grid, dim3(kWideBlock), 0, s, a);
+        }
     }
     return static_cast<int>(hipGetLastError());
 }

@@ -193,6 +193,18 @@ class Engine:
         self._check(self.lib.mpf_last_timing(self.h, C.byref(a), C.byref(b)), "mpf_last_timing")
         return a.value, b.value
 
+    def step_stats(self):
+        """Cumulative (mid-kernel, wide-kernel) cell-steps since creation."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.lib.mpf_step_stats(self.h, C.byref(a), C.byref(b)), "mpf_step_stats")
+        return a.value, b.value
+
+    def copy_bandwidth(self, nbytes: int, repeats: int = 20) -> float:
+        """Average time (ms) of the library's streaming copy kernel over `nbytes`."""
+        ms = C.c_double()
+        self._check(self.lib.mpf_copy_bandwidth(self.h, int(nbytes), int(repeats), C.byref(ms)), "mpf_copy_bandwidth")
+        return ms.value
+
     def device_state(self):
         cnt, idx, val, stream = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
         self._check(self.lib.mpf_device_state(self.h, C.byref(cnt), C.byref(idx), C.byref(val), C.byref(stream)),

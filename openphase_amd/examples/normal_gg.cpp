@@ -1,0 +1,50 @@
+// Normal grain growth through the OpenPhase-style C++ interface (include/openphase_amd/OpenPhaseAMD.h):
+// the time loop of the reference's examples/NormalGG/NormalGG.cpp:74-85, unchanged apart from the
+// namespace, on a microstructure read from a raw PhaseField .dat file.
+//
+// usage: normal_gg <input.opi> <state.dat> <ngrains> <nsteps> <out.dat>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "openphase_amd/OpenPhaseAMD.h"
+
+using namespace openphase_amd;
+
+int main(int argc, char* argv[])
+{
+    if (argc < 6) {
+        std::fprintf(stderr, "usage: %s input.opi state.dat ngrains nsteps out.dat\n", argv[0]);
+        return 2;
+    }
+    const std::string InputFile = argv[1];
+    const int ngrains = std::atoi(argv[3]);
+    const int nsteps = std::atoi(argv[4]);
+    try {
+        Settings OPSettings;
+        OPSettings.ReadInput(InputFile);
+        PhaseField Phi(OPSettings, InputFile);
+        DoubleObstacle DO(OPSettings, InputFile);
+        InterfaceProperties IP(OPSettings, InputFile);
+        BoundaryConditions BC(OPSettings, InputFile);
+        for (int g = 0; g < ngrains; ++g) Phi.AddGrainInfo(0);
+        if (!Phi.Read(argv[2])) {
+            std::fprintf(stderr, "could not read %s\n", argv[2]);
+            return 3;
+        }
+        Phi.FinalizeInitialization(BC);
+        const double dt = IP.ReportMaximumTimeStep();
+        for (int tStep = 1; tStep <= nsteps; ++tStep) {
+            IP.Set(Phi, BC);
+            DO.CalculatePhaseFieldIncrements(Phi, IP);
+            Phi.NormalizeIncrements(BC, dt);
+            Phi.MergeIncrements(BC, dt);
+        }
+        if (!Phi.Write(argv[5])) return 4;
+        std::printf("normal_gg: %d steps at dt=%.17g\n", nsteps, dt);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "normal_gg: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

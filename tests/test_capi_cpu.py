@@ -1,0 +1,54 @@
+"""The C ABI library loads on a machine without a GPU and exports every entry point of
+include/mpf.h (no compute calls here)."""
+import re
+from pathlib import Path
+
+import pytest
+
+from openphase_amd import _capi
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    text = (ROOT / "include" / "mpf.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpf_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declarations_are_known():
+    assert set(_declared()) == set(_capi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    if not _capi.LIB_PATH.exists():
+        pytest.skip("libmpf_hip.so not built")
+    lib = _capi.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_cpp_interface_compiles():
+    """The OpenPhase-style header and the NormalGG example compile with the host compiler."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("no g++")
+    res = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}",
+                          str(ROOT / "openphase_amd" / "examples" / "normal_gg.cpp")], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+
+
+def test_invalid_arguments_fail_without_a_gpu():
+    """Argument checks run before any device call: a null grid is MPF_ERR_ARG."""
+    if not _capi.LIB_PATH.exists():
+        pytest.skip("libmpf_hip.so not built")
+    import ctypes as C
+
+    lib = _capi.load()
+    h = C.c_void_p()
+    assert lib.mpf_create(None, None, C.byref(h)) == 1
+    assert lib.mpf_step(None, 1e-3, 1) == 1
+    assert lib.mpf_error_message(None) == b"null context"
