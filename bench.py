@@ -35,6 +35,7 @@ SEED = 0x0FE2
 # stored-list capacity: 8 overflows in the first steps of this 3D case (9 grains meet in a few
 # cells); only occupied slots are read or written, so the capacity does not change the traffic
 SLOTS = 16
+SLOTS_FP32 = 15   # the FP32 count byte holds the list length in 4 bits beside the implicit slot
 
 
 def _device_sync():
@@ -53,7 +54,7 @@ def algorithmic_bytes(count_in: np.ndarray, count_out: np.ndarray, value_bytes: 
 def run_gpu(precision: int, lists, steps: int, warmup: int):
     from openphase_amd.engine import Engine, StepConfig
 
-    cfg = StepConfig(NX, NY, NZ, precision=precision, slots=SLOTS)
+    cfg = StepConfig(NX, NY, NZ, precision=precision, slots=SLOTS if precision == 0 else SLOTS_FP32)
     dt = cfg.max_time_step()
     eng = Engine(cfg)
     eng.set_grains(np.zeros(NGRAINS, np.int32))

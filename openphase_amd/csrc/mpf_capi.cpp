@@ -205,6 +205,7 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         return bail(MPF_ERR_ARG, "grid sizes must be >= 0 and not all zero");
     if (!(grid->dx > 0.0) || !(grid->iwidth > 0.0)) return bail(MPF_ERR_ARG, "dx and iwidth must be positive");
     if (grid->slots < 2 || grid->slots > mpf::kMaxSlots) return bail(MPF_ERR_ARG, "slots must be in [2, 16]");
+    if (grid->precision == MPF_FP32 && grid->slots > 15) return bail(MPF_ERR_ARG, "FP32 states hold at most 15 slots");
     if (grid->precision != MPF_FP64 && grid->precision != MPF_FP32) return bail(MPF_ERR_ARG, "unknown precision");
     for (int i = 0; i < 6; ++i)
         if (grid->bc[i] != MPF_BC_PERIODIC) return bail(MPF_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");
@@ -399,7 +400,8 @@ mpf_status mpf_upload(mpf_ctx* c, const int32_t* count, const int32_t* index, co
         if (count[i] < 1 || count[i] > K)
             return c->fail(count[i] < 1 ? MPF_ERR_ARG : MPF_SLOT_OVERFLOW, "cell " + std::to_string(i) + " has " +
                            std::to_string(count[i]) + " entries (allowed 1.." + std::to_string(K) + ")");
-        cnt[i] = static_cast<uint8_t>(count[i]);
+        // FP32 states: implicit slot 15 = every value stored (FinalizeInitialization re-encodes)
+        cnt[i] = static_cast<uint8_t>(c->fp32 ? (count[i] | (15 << 4)) : count[i]);
         for (int s = 0; s < count[i]; ++s, ++off) {
             if (index[off] < 0 || index[off] >= c->ngrains)
                 return c->fail(MPF_ERR_ARG, "phase-field index out of range in cell " + std::to_string(i));
@@ -501,7 +503,7 @@ mpf_status mpf_total_entries(mpf_ctx* c, int64_t* total)
     if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     if (s) return s;
     int64_t t = 0;
-    for (uint8_t v : cnt) t += v;
+    for (uint8_t v : cnt) t += c->fp32 ? (v & 15) : v;
     *total = t;
     return MPF_OK;
 }
@@ -524,17 +526,26 @@ mpf_status mpf_download(mpf_ctx* c, int32_t* count, int32_t* index, double* valu
                                       N * K * c->vsize(), hipMemcpyDeviceToHost, c->stream), "download");
     if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     if (s) return s;
+    // decode FP32 states: count | (implicit slot << 4); the implicit value is 1 - sum(others)
+    std::vector<uint8_t> n(N);
+    for (size_t i = 0; i < N; ++i) n[i] = c->fp32 ? (cnt[i] & 15) : cnt[i];
     size_t off = 0;
     for (size_t i = 0; i < N; ++i) {
-        if (count) count[i] = cnt[i];
-        for (int q = 0; q < cnt[i]; ++q, ++off) {
+        if (count) count[i] = n[i];
+        const int imp = c->fp32 && (cnt[i] >> 4) != 15 ? (cnt[i] >> 4) : -1;
+        double others = 0.0;
+        for (int q = 0; q < n[i]; ++q)
+            if (q != imp) others += c->fp32 ? static_cast<double>(vf[static_cast<size_t>(q) * N + i]) : vd[static_cast<size_t>(q) * N + i];
+        for (int q = 0; q < n[i]; ++q, ++off) {
             if (index) index[off] = ids[static_cast<size_t>(q) * N + i];
-            if (value) value[off] = c->fp32 ? static_cast<double>(vf[static_cast<size_t>(q) * N + i]) : vd[static_cast<size_t>(q) * N + i];
+            if (!value) continue;
+            if (q == imp) value[off] = 1.0 - others;
+            else value[off] = c->fp32 ? static_cast<double>(vf[static_cast<size_t>(q) * N + i]) : vd[static_cast<size_t>(q) * N + i];
         }
     }
     if (flag) {
         for (size_t i = 0; i < N; ++i) {
-            if (cnt[i] > 1) { flag[i] = 2; continue; }
+            if (n[i] > 1) { flag[i] = 2; continue; }
             const int x = static_cast<int>(i / (static_cast<size_t>(c->ny) * c->nz));
             const int y = static_cast<int>((i / c->nz) % c->ny);
             const int z = static_cast<int>(i % c->nz);
@@ -544,7 +555,7 @@ mpf_status mpf_download(mpf_ctx* c, int32_t* count, int32_t* index, double* valu
                     for (int kk = -c->dnz; kk <= c->dnz && !f; ++kk) {
                         const int xx = (x + ii + c->nx) % c->nx, yy = (y + jj + c->ny) % c->ny, zz = (z + kk + c->nz) % c->nz;
                         const size_t nb = (static_cast<size_t>(xx) * c->ny + yy) * c->nz + zz;
-                        if (cnt[nb] > 1) f = 1;
+                        if (n[nb] > 1) f = 1;
                     }
             flag[i] = f;
         }

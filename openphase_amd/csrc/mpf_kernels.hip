@@ -149,6 +149,27 @@ __device__ __forceinline__ TC cubic_energy(const double* R, TC gx, TC gy, TC gz,
 
 enum CellResult { kCellBulk = 0, kCellWide = 1, kCellDeferred = 2, kCellOverflow = 3 };
 
+// Count byte of a stored list. FP64 states store the entry count. FP32 states store
+// count | (implicit slot << 4): the largest entry's value is not stored but recovered as
+// 1 - sum(other entries) in FP64, so values near 1 keep their FP64 resolution and single-entry
+// cells store no value at all.
+// Implicit slot 15 means "none" (every value stored): the form mpf_upload writes.
+template <typename TV> struct ListCode {
+    static constexpr bool kImplicit = sizeof(TV) == 4;
+    __device__ static int count(uint8_t b) { return kImplicit ? (b & 15) : b; }
+    __device__ static int implicit(uint8_t b) { return kImplicit && (b >> 4) != 15 ? (b >> 4) : -1; }
+};
+
+// FP64 value of the implicit entry of a stored list of m entries.
+template <typename TV>
+__device__ __forceinline__ double implicit_value(const StepArgs& A, int64_t cell, int m, int imp)
+{
+    double s = 0.0;
+    for (int q = 0; q < m; ++q)
+        if (q != imp) s += static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * A.ncells + cell));
+    return 1.0 - s;
+}
+
 // Per-cell list held in registers (K entries): phase-field index, value (FP64 and in the stencil
 // arithmetic), Laplacian and gradient of the neighbour-merged list.
 template <int K, typename TC, bool GRAD> struct CellList {
@@ -174,7 +195,9 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
     const int x = static_cast<int>(cc / (static_cast<int64_t>(NY) * NZ));
     const int y = static_cast<int>((cc / NZ) % NY);
     const int z = static_cast<int>(cc % NZ);
-    const int n0 = A.cnt_in[cc];
+    const uint8_t code = A.cnt_in[cc];
+    const int n0 = ListCode<TV>::count(code);
+    const int imp0 = ListCode<TV>::implicit(code);
     L.n0 = n0;
     L.ne = n0;
 #pragma unroll U
@@ -183,15 +206,24 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
         if (GRAD) { L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][2] = TC(0); }
     }
     if (n0 > K) return kCellDeferred;
+    double others = 0.0;
 #pragma unroll U
     for (int t = 0; t < K; ++t)
         if (t < n0) {
-            const TV v = load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc);
             L.eid[t] = A.id_in[static_cast<int64_t>(t) * N + cc];
-            L.phi[t] = static_cast<double>(v);
-            L.ev[t] = static_cast<TC>(v);
             L.keep[t] = true;
+            if (!(ListCode<TV>::kImplicit && t == imp0)) {
+                L.phi[t] = static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc));
+                others += L.phi[t];
+            }
         }
+    if (ListCode<TV>::kImplicit) {
+#pragma unroll U
+        for (int t = 0; t < K; ++t)
+            if (t == imp0) L.phi[t] = 1.0 - others;
+    }
+#pragma unroll U
+    for (int t = 0; t < K; ++t) L.ev[t] = static_cast<TC>(L.phi[t]);
 
     // neighbour coordinates (periodic wrap)
     int xs[3], ys[3], zs[3];
@@ -209,7 +241,7 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
             for (int jj = -A.dny; jj <= A.dny; ++jj)
                 for (int kk = -A.dnz; kk <= A.dnz; ++kk) {
                     const int64_t nb = (static_cast<int64_t>(xs[ii + 1]) * NY + ys[jj + 1]) * NZ + zs[kk + 1];
-                    wide = wide || A.cnt_in[nb] > 1;
+                    wide = wide || ListCode<TV>::count(A.cnt_in[nb]) > 1;
                 }
     }
     if (!wide) return kCellBulk;
@@ -225,9 +257,14 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
                 const TC gwx = ii == 0 ? TC(0) : (ii < 0 ? -gw : gw);
                 const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
                 const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
-                const int m = A.cnt_in[nb];
+                const uint8_t nbcode = A.cnt_in[nb];
+                const int m = ListCode<TV>::count(nbcode);
+                const int imp = ListCode<TV>::implicit(nbcode);
+                const double vimp = ListCode<TV>::kImplicit ? implicit_value<TV>(A, nb, m, imp) : 0.0;
                 for (int q = 0; q < m; ++q) {
-                    const TC gv = static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
+                    const TC gv = (ListCode<TV>::kImplicit && q == imp)
+                                      ? static_cast<TC>(vimp)
+                                      : static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
                     if (gv == TC(0)) continue;
                     const uint32_t gid = A.id_in[static_cast<int64_t>(q) * N + nb];
                     const TC l = w * gv;
@@ -585,22 +622,41 @@ template <int K, typename TV, typename TC, bool GRAD>
 __device__ __forceinline__ bool store_cell(const StepArgs& A, int64_t cc, CellList<K, TC, GRAD>& L)
 {
     constexpr int U = K <= kFastSlots ? K : 1;
+    constexpr bool IMPL = ListCode<TV>::kImplicit;
     const int64_t N = A.ncells;
     int ns = 0;
-#pragma unroll U
-    for (int t = 0; t < K; ++t) ns += L.keep[t] ? 1 : 0;
-    if (ns > A.slots) return false;
-    int w = 0;
+    int imp_t = -1;     // implicit entry (first largest value) by list position
+    double vmax = -1.0;
 #pragma unroll U
     for (int t = 0; t < K; ++t)
         if (L.keep[t]) {
-            const TV sv = static_cast<TV>(L.phi[t]);
-            L.phi[t] = static_cast<double>(sv);
+            ns += 1;
+            if (IMPL && L.phi[t] > vmax) { vmax = L.phi[t]; imp_t = t; }
+        }
+    if (ns > A.slots) return false;
+    int w = 0, imp_w = 0;
+    double others = 0.0;
+#pragma unroll U
+    for (int t = 0; t < K; ++t)
+        if (L.keep[t]) {
             A.id_out[static_cast<int64_t>(w) * N + cc] = static_cast<uint16_t>(L.eid[t]);
-            static_cast<TV*>(A.val_out)[static_cast<int64_t>(w) * N + cc] = sv;
+            if (IMPL && t == imp_t) {
+                imp_w = w;
+            } else {
+                const TV sv = static_cast<TV>(L.phi[t]);
+                L.phi[t] = static_cast<double>(sv);
+                others += L.phi[t];
+                static_cast<TV*>(A.val_out)[static_cast<int64_t>(w) * N + cc] = sv;
+            }
             ++w;
         }
-    A.cnt_out[cc] = static_cast<uint8_t>(w);
+    if (IMPL) {
+        // the value the next step will recover (same summation order: slot order)
+#pragma unroll U
+        for (int t = 0; t < K; ++t)
+            if (t == imp_t) L.phi[t] = 1.0 - others;
+    }
+    A.cnt_out[cc] = static_cast<uint8_t>(IMPL ? (w | (imp_w << 4)) : w);
     return true;
 }
 
@@ -728,17 +784,24 @@ __global__ __launch_bounds__(kBlock) void finalize_init_kernel(const StepArgs A)
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const bool valid = c < N;
     const int64_t cc = valid ? c : 0;
-    const int n0 = valid ? A.cnt_in[cc] : 0;
+    const uint8_t code = valid ? A.cnt_in[cc] : 0;
+    const int n0 = ListCode<TV>::count(code);
+    const int imp0 = ListCode<TV>::implicit(code);
     CellList<K, double, false> L;
     bool bad_volume = false;
+    double others = 0.0;
     for (int t = 0; t < K; ++t) {
         L.eid[t] = 0u; L.phi[t] = 0.0; L.keep[t] = false;
         if (t < n0) {
             L.eid[t] = A.id_in[static_cast<int64_t>(t) * N + cc];
-            L.phi[t] = static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc));
+            if (t != imp0) {
+                L.phi[t] = static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc));
+                others += L.phi[t];
+            }
             L.keep[t] = true;
         }
     }
+    if (imp0 >= 0 && imp0 < n0) L.phi[imp0] = 1.0 - others;
     if (valid && n0 > 1) {
         double total = 0.0;
         int ns = 0;
@@ -760,20 +823,8 @@ __global__ __launch_bounds__(kBlock) void finalize_init_kernel(const StepArgs A)
             for (int t = 0; t < n0; ++t) if (L.keep[t]) L.phi[t] = 1.0;
         }
     }
-    if (valid) {
-        // TV storage; the list was validated against the capacity on upload
-        const int64_t NN = N;
-        int w = 0;
-        for (int t = 0; t < K; ++t)
-            if (L.keep[t]) {
-                const TV sv = static_cast<TV>(L.phi[t]);
-                L.phi[t] = static_cast<double>(sv);
-                A.id_out[static_cast<int64_t>(w) * NN + cc] = static_cast<uint16_t>(L.eid[t]);
-                static_cast<TV*>(A.val_out)[static_cast<int64_t>(w) * NN + cc] = sv;
-                ++w;
-            }
-        A.cnt_out[cc] = static_cast<uint8_t>(w);
-    }
+    // the list was validated against the capacity on upload
+    if (valid) store_cell<K, TV, double, false>(A, cc, L);
     for (int t = 0; t < K; ++t) wave_volume_add<true>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
     if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
 }

@@ -78,8 +78,18 @@ def test_fp64_matches_oracle(shape, ngrains, steps):
     eng.close()
 
 
+def _dense(lists, ng):
+    out = np.zeros((ng, lists.ncells))
+    cell = np.repeat(np.arange(lists.ncells), lists.count)
+    out[lists.index, cell] = lists.value
+    return out
+
+
 def test_fp32_error_vs_fp64():
-    """FP32 storage/stencils with FP64 limiter: bounded phase-fraction error vs the FP64 oracle."""
+    """FP32 storage (largest entry of each cell implicit, 1 - sum of the others in FP64) with the
+    FP64 limiter: maximum phase-fraction error against the FP64 oracle after 10, 100 and 1000
+    steps of a 64x64 grain-growth run (measured: 1.2e-7, 2.2e-7, 1.3e-5 with FP32 stencils;
+    8.7e-7 after 1000 steps with FP64 stencils); every state sums to one."""
     cfg64 = StepConfig(64, 64, 0)
     cfg32 = StepConfig(64, 64, 0, precision=_capi.MPF_FP32)
     lists = voronoi_grains(64, 64, 0, 16, seed=5)
@@ -91,24 +101,21 @@ def test_fp32_error_vs_fp64():
         e.set_grains(phase)
         e.upload(lists)
         e.finalize_initialization()
-    eng.step(dt, 100)
-    orc.step(dt, 100)
-    a, b = eng.download(), orc.download()
-    ids_a, va = a.to_slots(8)
-    ids_b, vb = b.to_slots(8)
-    # dense per-grain fields for a set-independent comparison
-    def dense(lists, ng):
-        out = np.zeros((ng, lists.ncells))
-        cell = np.repeat(np.arange(lists.ncells), lists.count)
-        out[lists.index, cell] = lists.value
-        return out
-    err = np.abs(dense(a, 16) - dense(b, 16)).max()
-    print("fp32 max phase-fraction error after 100 steps", err)
-    # bound of the current FP32 path (values rounded to float between steps)
-    assert err < 5e-3
+    done = 0
+    for steps, bound in ((10, 1e-6), (100, 2e-6), (1000, 5e-5)):
+        eng.step(dt, steps - done)
+        orc.step(dt, steps - done)
+        done = steps
+        a, b = eng.download(), orc.download()
+        err = np.abs(_dense(a, 16) - _dense(b, 16)).max()
+        print(f"fp32 max phase-fraction error after {steps} steps: {err:.3e}")
+        assert err < bound
+        sums = np.add.reduceat(a.value, a.offsets()[:-1])
+        assert np.abs(sums - 1.0).max() < 1e-12
+        assert a.value.min() > 0.0 and a.value.max() <= 1.0
     va_, _ = eng.grains()
     vb_, _ = orc.grains()
-    np.testing.assert_allclose(va_, vb_, rtol=1e-4, atol=1e-2)
+    np.testing.assert_allclose(va_, vb_, rtol=1e-5, atol=1e-3)
     eng.close()
 
 
