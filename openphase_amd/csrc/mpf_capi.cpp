@@ -218,6 +218,9 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     c->nz = grid->nz ? grid->nz : 1;
     c->active = c->dnx + c->dny + c->dnz;
     c->ncells = static_cast<int64_t>(c->nx) * c->ny * c->nz;
+    // the step kernels address one slot plane of FP64 values with 31-bit byte offsets
+    if (c->ncells >= (int64_t{1} << 28))
+        return bail(MPF_ERR_UNSUPPORTED, "at most 2^28 - 1 cells per device (e.g. 645^3)");
     c->slots = grid->slots;
     c->fp32 = grid->precision == MPF_FP32;
     c->nphases = params->nphases;

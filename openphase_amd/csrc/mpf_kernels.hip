@@ -11,10 +11,13 @@
 // include/Containers/NodePF.h:214-255), and writes the next state. Periodic halos are wrapped
 // indices (BoundaryConditions::SetX/Y/Z, include/BoundaryConditions.h:694-712).
 //
-// Work per cell is dominated by the merged-list length n (2-4 in grain growth). The merge runs in
-// registers with capacity 8; the pair/limiter/finalize part is compiled separately for n = 2, 3, 4
-// and for n <= 8, so a wavefront only executes the sizes its cells have. Cells whose merged list is
-// longer than 8 are deferred to a work list finished by a second kernel (capacity 16).
+// Work per cell is dominated by the merged-list length n (2-4 in grain growth). The main kernel
+// merges in registers with capacity 4 and compiles the pair/limiter/finalize part separately for
+// n = 2, 3, 4, so a wavefront only executes the sizes its cells have. Cells with longer lists go
+// to a work list for a second kernel (n = 5, 6 in registers) and from there to a third (up to 16).
+// The neighbour lists are read with batched loads (all count bytes of the box, then the first two
+// entries of each plane of nine neighbours), so a wavefront waits on a few memory round trips per
+// cell rather than one per neighbour entry.
 //
 // Per-grain volume sums (PhaseField::CalculateGrainsVolume, src/PhaseField.cpp:812-951) are
 // reduced per wavefront in a fixed lane order and added across wavefronts as 128-bit fixed point
@@ -33,7 +36,8 @@ namespace mpf {
 
 namespace {
 
-constexpr int kFastSlots = 8;    // merged-list capacity of the register path
+constexpr int kMainSlots = 4;    // merged-list capacity of the main kernel
+constexpr int kFastSlots = 8;    // merged-list capacity of the register path of the mid kernel
 constexpr int kWideSlots = 16;   // merged-list capacity of the deferred path
 
 template <int K> __host__ __device__ constexpr int pair_index(int a, int b)
@@ -49,6 +53,34 @@ __device__ __forceinline__ int wrap1(int i, int n)
 template <typename T> __device__ __forceinline__ T load_val(const void* p, int64_t i)
 {
     return static_cast<const T*>(p)[i];
+}
+
+// Raw buffer loads over one slot plane (byte size < 2^31, checked by the host): an offset at or
+// beyond the plane size returns 0 without a memory access, so predicated neighbour loads need no
+// branch (CDNA buffer descriptor, word 3 = 0x00020000).
+constexpr uint32_t kNoLoad = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+
+template <typename T> __device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ uint8_t buffer_load<uint8_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+template <> __device__ __forceinline__ uint16_t buffer_load<uint16_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+template <> __device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
 // std::min / std::max semantics (first argument wins ties and unordered compares).
@@ -206,18 +238,20 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
         if (GRAD) { L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][2] = TC(0); }
     }
     if (n0 > K) return kCellDeferred;
-    double others = 0.0;
+    // own list: all loads first, then the implicit entry (1 - sum of the others in slot order)
 #pragma unroll U
     for (int t = 0; t < K; ++t)
         if (t < n0) {
             L.eid[t] = A.id_in[static_cast<int64_t>(t) * N + cc];
             L.keep[t] = true;
-            if (!(ListCode<TV>::kImplicit && t == imp0)) {
+            if (!(ListCode<TV>::kImplicit && t == imp0))
                 L.phi[t] = static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + cc));
-                others += L.phi[t];
-            }
         }
     if (ListCode<TV>::kImplicit) {
+        double others = 0.0;
+#pragma unroll U
+        for (int t = 0; t < K; ++t)
+            if (t < n0 && t != imp0) others += L.phi[t];
 #pragma unroll U
         for (int t = 0; t < K; ++t)
             if (t == imp0) L.phi[t] = 1.0 - others;
@@ -225,82 +259,135 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
 #pragma unroll U
     for (int t = 0; t < K; ++t) L.ev[t] = static_cast<TC>(L.phi[t]);
 
-    // neighbour coordinates (periodic wrap)
-    int xs[3], ys[3], zs[3];
+    // Neighbour cells as 32-bit offsets: nine row starts (ii, jj) plus the z coordinate. An
+    // offset along an inactive axis aliases the centre and is masked out (box position
+    // k = 9 (ii+1) + 3 (jj+1) + (kk+1) is stencil order: x outer, z inner).
+    const int xl = A.dnx ? wrap1(x - 1, A.nx) : x, xh = A.dnx ? wrap1(x + 1, A.nx) : x;
+    const int yl = A.dny ? wrap1(y - 1, NY) : y, yh = A.dny ? wrap1(y + 1, NY) : y;
+    const uint32_t zs[3] = {static_cast<uint32_t>(A.dnz ? wrap1(z - 1, NZ) : z), static_cast<uint32_t>(z),
+                            static_cast<uint32_t>(A.dnz ? wrap1(z + 1, NZ) : z)};
+    uint32_t rows[9];
+    {
+        const int xv[3] = {xl, x, xh}, yv[3] = {yl, y, yh};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        xs[d] = wrap1(x + d - 1, A.nx);
-        ys[d] = wrap1(y + d - 1, NY);
-        zs[d] = wrap1(z + d - 1, NZ);
+        for (int r = 0; r < 9; ++r)
+            rows[r] = static_cast<uint32_t>((xv[r / 3] * NY + yv[r % 3]) * NZ);
     }
+    auto active = [&](int k) {
+        const int ii = k / 9 - 1, jj = (k / 3) % 3 - 1, kk = k % 3 - 1;
+        return (ii == 0 || A.dnx) && (jj == 0 || A.dny) && (kk == 0 || A.dnz);
+    };
+    auto neighbour = [&](int k) -> uint32_t { return rows[k / 3] + zs[k % 3]; };
 
+    // All count bytes of the box first: 27 independent loads, packed four to a word.
     // wide_interface(): flag 2 (more than one entry) or a flag-2 cell in the 3^d box
-    bool wide = n0 > 1;
-    if (!wide) {
-        for (int ii = -A.dnx; ii <= A.dnx; ++ii)
-            for (int jj = -A.dny; jj <= A.dny; ++jj)
-                for (int kk = -A.dnz; kk <= A.dnz; ++kk) {
-                    const int64_t nb = (static_cast<int64_t>(xs[ii + 1]) * NY + ys[jj + 1]) * NZ + zs[kk + 1];
-                    wide = wide || ListCode<TV>::count(A.cnt_in[nb]) > 1;
-                }
+    const uint32_t N32 = static_cast<uint32_t>(N);
+    uint32_t codes[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    {
+        const __amdgpu_buffer_rsrc_t rc = plane_rsrc(A.cnt_in, N32);
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+            const uint8_t b = k == 13 ? code : buffer_load<uint8_t>(rc, active(k) ? neighbour(k) : kNoLoad);
+            codes[k >> 2] |= static_cast<uint32_t>(b) << (8 * (k & 3));
+        }
     }
+    bool wide = n0 > 1;
+#pragma unroll
+    for (int k = 0; k < 27; ++k)
+        wide = wide || ListCode<TV>::count(static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)))) > 1;
     if (!wide) return kCellBulk;
 
+    // Merge plane by plane (ii = -1, 0, 1): the first two entries of the plane's nine neighbours
+    // are loaded together (branch-free buffer loads), later entries (rare) one by one; entries
+    // are merged in stencil order.
+    const __amdgpu_buffer_rsrc_t ri0 = plane_rsrc(A.id_in, N32 * 2u);
+    const __amdgpu_buffer_rsrc_t ri1 = plane_rsrc(A.id_in + N, N32 * 2u);
+    const __amdgpu_buffer_rsrc_t rv0 = plane_rsrc(A.val_in, N32 * static_cast<uint32_t>(sizeof(TV)));
+    const __amdgpu_buffer_rsrc_t rv1 = plane_rsrc(static_cast<const TV*>(A.val_in) + N, N32 * static_cast<uint32_t>(sizeof(TV)));
     int ne = n0;
-    for (int ii = -A.dnx; ii <= A.dnx; ++ii)
-        for (int jj = -A.dny; jj <= A.dny; ++jj)
-            for (int kk = -A.dnz; kk <= A.dnz; ++kk) {
-                const int64_t nb = (static_cast<int64_t>(xs[ii + 1]) * NY + ys[jj + 1]) * NZ + zs[kk + 1];
-                const int nnz = (ii != 0) + (jj != 0) + (kk != 0);
-                const TC w = static_cast<TC>(A.wl[nnz]);
-                const TC gw = static_cast<TC>(A.wg[nnz]);
-                const TC gwx = ii == 0 ? TC(0) : (ii < 0 ? -gw : gw);
-                const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
-                const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
-                const uint8_t nbcode = A.cnt_in[nb];
-                const int m = ListCode<TV>::count(nbcode);
-                const int imp = ListCode<TV>::implicit(nbcode);
-                const double vimp = ListCode<TV>::kImplicit ? implicit_value<TV>(A, nb, m, imp) : 0.0;
-                for (int q = 0; q < m; ++q) {
-                    const TC gv = (ListCode<TV>::kImplicit && q == imp)
-                                      ? static_cast<TC>(vimp)
-                                      : static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
-                    if (gv == TC(0)) continue;
-                    const uint32_t gid = A.id_in[static_cast<int64_t>(q) * N + nb];
-                    const TC l = w * gv;
-                    const TC lx = GRAD && nnz ? gwx * gv : TC(0);
-                    const TC ly = GRAD && nnz ? gwy * gv : TC(0);
-                    const TC lz = GRAD && nnz ? gwz * gv : TC(0);
-                    bool found = false;
-                    // branch-free: adding +0 leaves a sum unchanged (the sums are never -0)
+#pragma unroll
+    for (int ip = 0; ip < 3; ++ip) {
+        if (ip != 1 && !A.dnx) continue;
+        TV v0[9], v1[9];
+        uint32_t g0[9], g1[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int k = 9 * ip + j;
+            const uint8_t b = static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)));
+            const int m = ListCode<TV>::count(b), imp = ListCode<TV>::implicit(b);
+            const uint32_t nb = neighbour(k);
+            g0[j] = buffer_load<uint16_t>(ri0, m > 0 ? nb * 2u : kNoLoad);
+            v0[j] = buffer_load<TV>(rv0, m > 0 && imp != 0 ? nb * static_cast<uint32_t>(sizeof(TV)) : kNoLoad);
+            g1[j] = buffer_load<uint16_t>(ri1, m > 1 ? nb * 2u : kNoLoad);
+            v1[j] = buffer_load<TV>(rv1, m > 1 && imp != 1 ? nb * static_cast<uint32_t>(sizeof(TV)) : kNoLoad);
+        }
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int k = 9 * ip + j;
+            const int ii = ip - 1, jj = j / 3 - 1, kk = j % 3 - 1;
+            const int64_t nb = neighbour(k);
+            const int nnz = (ii != 0) + (jj != 0) + (kk != 0);
+            const TC w = static_cast<TC>(A.wl[nnz]);
+            const TC gw = static_cast<TC>(A.wg[nnz]);
+            const TC gwx = ii == 0 ? TC(0) : (ii < 0 ? -gw : gw);
+            const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
+            const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
+            const uint8_t nbcode = static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)));
+            const int m = ListCode<TV>::count(nbcode);
+            const int imp = ListCode<TV>::implicit(nbcode);
+            // implicit entry: 1 - sum of the others in slot order
+            double vimp = 0.0;
+            if (ListCode<TV>::kImplicit && imp >= 0 && m > 0) {
+                if (m == 1) vimp = 1.0;
+                else if (m == 2) vimp = 1.0 - (0.0 + static_cast<double>(imp == 0 ? v1[j] : v0[j]));
+                else vimp = implicit_value<TV>(A, nb, m, imp);
+            }
+            for (int q = 0; q < m; ++q) {
+                TC gv;
+                uint32_t gid;
+                if (ListCode<TV>::kImplicit && q == imp) gv = static_cast<TC>(vimp);
+                else if (q == 0) gv = static_cast<TC>(v0[j]);
+                else if (q == 1) gv = static_cast<TC>(v1[j]);
+                else gv = static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
+                if (gv == TC(0)) continue;
+                if (q == 0) gid = g0[j];
+                else if (q == 1) gid = g1[j];
+                else gid = A.id_in[static_cast<int64_t>(q) * N + nb];
+                const TC l = w * gv;
+                const TC lx = GRAD && nnz ? gwx * gv : TC(0);
+                const TC ly = GRAD && nnz ? gwy * gv : TC(0);
+                const TC lz = GRAD && nnz ? gwz * gv : TC(0);
+                bool found = false;
+                // branch-free: adding +0 leaves a sum unchanged (the sums are never -0)
 #pragma unroll U
-                    for (int t = 0; t < K; ++t) {
-                        const bool mt = t < ne && L.eid[t] == gid;
-                        L.lap[t] += mt ? l : TC(0);
-                        if (GRAD && nnz) {
-                            L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][0] + (mt ? lx : TC(0));
-                            L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][1] + (mt ? ly : TC(0));
-                            L.gr[GRAD ? t : 0][2] = L.gr[GRAD ? t : 0][2] + (mt ? lz : TC(0));
-                        }
-                        found = found || mt;
+                for (int t = 0; t < K; ++t) {
+                    const bool mt = t < ne && L.eid[t] == gid;
+                    L.lap[t] += mt ? l : TC(0);
+                    if (GRAD && nnz) {
+                        L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][0] + (mt ? lx : TC(0));
+                        L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][1] + (mt ? ly : TC(0));
+                        L.gr[GRAD ? t : 0][2] = L.gr[GRAD ? t : 0][2] + (mt ? lz : TC(0));
                     }
-                    if (!found) {
-                        if (ne >= K) return kCellDeferred;
+                    found = found || mt;
+                }
+                if (!found) {
+                    if (ne >= K) return kCellDeferred;
 #pragma unroll U
-                        for (int t = 0; t < K; ++t)
-                            if (t == ne) {
-                                L.eid[t] = gid;
-                                L.lap[t] = l;
-                                if (GRAD) {
-                                    L.gr[GRAD ? t : 0][0] = lx;
-                                    L.gr[GRAD ? t : 0][1] = ly;
-                                    L.gr[GRAD ? t : 0][2] = lz;
-                                }
+                    for (int t = 0; t < K; ++t)
+                        if (t == ne) {
+                            L.eid[t] = gid;
+                            L.lap[t] = l;
+                            if (GRAD) {
+                                L.gr[GRAD ? t : 0][0] = lx;
+                                L.gr[GRAD ? t : 0][1] = ly;
+                                L.gr[GRAD ? t : 0][2] = lz;
                             }
-                        ++ne;
-                    }
+                        }
+                    ++ne;
                 }
             }
+        }
+    }
     L.ne = ne;
     return kCellWide;
 }
@@ -668,11 +755,21 @@ __device__ __forceinline__ void report_overflow(const StepArgs& A, int64_t c, Ce
     atomicMin(A.first_overflow, static_cast<unsigned long long>(c));
 }
 
+// Workgroups are dispatched round-robin over the eight XCDs, each with its own L2. Renumbering
+// them so that every XCD walks one contiguous range of cells keeps the neighbour rows a block
+// reads mostly in the L2 that already holds them. A bijection of [0, nb).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+    constexpr unsigned kXcds = 8;
+    const unsigned q = nb / kXcds, r = nb % kXcds, x = b % kXcds, i = b / kXcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 template <typename TV, typename TC, bool GRAD, bool DG>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs A)
 {
-    constexpr int K = kFastSlots;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    constexpr int K = kMainSlots;
+    const int64_t c = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
     const bool valid = c < A.ncells;
     CellList<K, TC, GRAD> L;
     bool empty = false, bad_volume = false;
