@@ -215,19 +215,69 @@ template <int K, typename TC, bool GRAD> struct CellList {
     int ne;  // merged entries
 };
 
-// Loads the own list and, for wide cells, merges the neighbour lists in stencil order (x outer,
-// z inner) while accumulating Laplacians/gradients. Returns kCellBulk (nothing to do),
-// kCellWide, or kCellDeferred when more than K entries are needed.
-template <int K, typename TV, typename TC, bool GRAD>
-__device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellList<K, TC, GRAD>& L)
+// Neighbour cells of one cell as 32-bit offsets: nine row starts (ii, jj) plus the z coordinate.
+// An offset along an inactive axis aliases the centre and is masked out. Box position
+// k = 9 (ii+1) + 3 (jj+1) + (kk+1) is stencil order (x outer, z inner).
+struct Box {
+    uint32_t rows[9];
+    uint32_t zs[3];
+    __device__ uint32_t cell(int k) const { return rows[k / 3] + zs[k % 3]; }
+};
+
+__device__ __forceinline__ Box make_box(const StepArgs& A, int64_t cc)
 {
-    constexpr int U = K <= kFastSlots ? K : 1;  // full unroll only in registers
-    const int64_t N = A.ncells;
     const int NY = A.ny, NZ = A.nz;
     const int x = static_cast<int>(cc / (static_cast<int64_t>(NY) * NZ));
     const int y = static_cast<int>((cc / NZ) % NY);
     const int z = static_cast<int>(cc % NZ);
-    const uint8_t code = A.cnt_in[cc];
+    Box B;
+    const int xv[3] = {A.dnx ? wrap1(x - 1, A.nx) : x, x, A.dnx ? wrap1(x + 1, A.nx) : x};
+    const int yv[3] = {A.dny ? wrap1(y - 1, NY) : y, y, A.dny ? wrap1(y + 1, NY) : y};
+    B.zs[0] = static_cast<uint32_t>(A.dnz ? wrap1(z - 1, NZ) : z);
+    B.zs[1] = static_cast<uint32_t>(z);
+    B.zs[2] = static_cast<uint32_t>(A.dnz ? wrap1(z + 1, NZ) : z);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) B.rows[r] = static_cast<uint32_t>((xv[r / 3] * NY + yv[r % 3]) * NZ);
+    return B;
+}
+
+__device__ __forceinline__ bool box_active(const StepArgs& A, int k)
+{
+    const int ii = k / 9 - 1, jj = (k / 3) % 3 - 1, kk = k % 3 - 1;
+    return (ii == 0 || A.dnx) && (jj == 0 || A.dny) && (kk == 0 || A.dnz);
+}
+
+__device__ __forceinline__ uint8_t box_code(const uint32_t codes[7], int k)
+{
+    return static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)));
+}
+
+// Count bytes of the whole box (27 independent loads, packed four to a word; inactive positions
+// read as 0). Returns wide_interface(): flag 2 (more than one entry) or a flag-2 cell in the box.
+template <typename TV>
+__device__ __forceinline__ bool load_codes(const StepArgs& A, const Box& B, uint8_t code, uint32_t codes[7])
+{
+    const __amdgpu_buffer_rsrc_t rc = plane_rsrc(A.cnt_in, static_cast<uint32_t>(A.ncells));
+#pragma unroll
+    for (int w = 0; w < 7; ++w) codes[w] = 0u;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const uint8_t b = k == 13 ? code : buffer_load<uint8_t>(rc, box_active(A, k) ? B.cell(k) : kNoLoad);
+        codes[k >> 2] |= static_cast<uint32_t>(b) << (8 * (k & 3));
+    }
+    bool wide = false;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) wide = wide || ListCode<TV>::count(box_code(codes, k)) > 1;
+    return wide;
+}
+
+// Loads the own list of cell cc into L (values in FP64; the implicit FP32 entry is 1 - sum of the
+// others in slot order). Returns kCellDeferred when it has more than K entries.
+template <int K, typename TV, typename TC, bool GRAD>
+__device__ __forceinline__ int load_own(const StepArgs& A, int64_t cc, uint8_t code, CellList<K, TC, GRAD>& L)
+{
+    constexpr int U = K <= kFastSlots ? K : 1;  // full unroll only in registers
+    const int64_t N = A.ncells;
     const int n0 = ListCode<TV>::count(code);
     const int imp0 = ListCode<TV>::implicit(code);
     L.n0 = n0;
@@ -238,7 +288,6 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
         if (GRAD) { L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][2] = TC(0); }
     }
     if (n0 > K) return kCellDeferred;
-    // own list: all loads first, then the implicit entry (1 - sum of the others in slot order)
 #pragma unroll U
     for (int t = 0; t < K; ++t)
         if (t < n0) {
@@ -258,144 +307,169 @@ __device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellLi
     }
 #pragma unroll U
     for (int t = 0; t < K; ++t) L.ev[t] = static_cast<TC>(L.phi[t]);
+    return kCellWide;
+}
 
-    // Neighbour cells as 32-bit offsets: nine row starts (ii, jj) plus the z coordinate. An
-    // offset along an inactive axis aliases the centre and is masked out (box position
-    // k = 9 (ii+1) + 3 (jj+1) + (kk+1) is stencil order: x outer, z inner).
-    const int xl = A.dnx ? wrap1(x - 1, A.nx) : x, xh = A.dnx ? wrap1(x + 1, A.nx) : x;
-    const int yl = A.dny ? wrap1(y - 1, NY) : y, yh = A.dny ? wrap1(y + 1, NY) : y;
-    const uint32_t zs[3] = {static_cast<uint32_t>(A.dnz ? wrap1(z - 1, NZ) : z), static_cast<uint32_t>(z),
-                            static_cast<uint32_t>(A.dnz ? wrap1(z + 1, NZ) : z)};
-    uint32_t rows[9];
-    {
-        const int xv[3] = {xl, x, xh}, yv[3] = {yl, y, yh};
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-            rows[r] = static_cast<uint32_t>((xv[r / 3] * NY + yv[r % 3]) * NZ);
+// Adds one neighbour entry (grain gid, weighted value l and gradient parts) to the merged list:
+// accumulate into the matching entry or append it (CalculateDerivativesSR appends neighbour
+// indices at value 0). Straight-line selects, no branches; `over` records an append beyond K.
+template <int K, typename TC, bool GRAD>
+__device__ __forceinline__ void merge_entry(CellList<K, TC, GRAD>& L, int& ne, bool& over, bool valid,
+                                            uint32_t gid, TC l, TC lx, TC ly, TC lz, bool side)
+{
+    constexpr int U = K <= kFastSlots ? K : 1;
+    bool found = false;
+    // adding +0 leaves a sum unchanged (the sums are never -0)
+#pragma unroll U
+    for (int t = 0; t < K; ++t) {
+        const bool mt = valid && t < ne && L.eid[t] == gid;
+        L.lap[t] += mt ? l : TC(0);
+        if (GRAD && side) {
+            L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][0] + (mt ? lx : TC(0));
+            L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][1] + (mt ? ly : TC(0));
+            L.gr[GRAD ? t : 0][2] = L.gr[GRAD ? t : 0][2] + (mt ? lz : TC(0));
+        }
+        found = found || mt;
     }
-    auto active = [&](int k) {
-        const int ii = k / 9 - 1, jj = (k / 3) % 3 - 1, kk = k % 3 - 1;
-        return (ii == 0 || A.dnx) && (jj == 0 || A.dny) && (kk == 0 || A.dnz);
-    };
-    auto neighbour = [&](int k) -> uint32_t { return rows[k / 3] + zs[k % 3]; };
-
-    // All count bytes of the box first: 27 independent loads, packed four to a word.
-    // wide_interface(): flag 2 (more than one entry) or a flag-2 cell in the 3^d box
-    const uint32_t N32 = static_cast<uint32_t>(N);
-    uint32_t codes[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
-    {
-        const __amdgpu_buffer_rsrc_t rc = plane_rsrc(A.cnt_in, N32);
-#pragma unroll
-        for (int k = 0; k < 27; ++k) {
-            const uint8_t b = k == 13 ? code : buffer_load<uint8_t>(rc, active(k) ? neighbour(k) : kNoLoad);
-            codes[k >> 2] |= static_cast<uint32_t>(b) << (8 * (k & 3));
+    const bool app = valid && !found;
+    over = over || (app && ne >= K);
+#pragma unroll U
+    for (int t = 0; t < K; ++t) {
+        const bool at = app && t == ne;
+        L.eid[t] = at ? gid : L.eid[t];
+        L.lap[t] = at ? l : L.lap[t];
+        if (GRAD) {
+            L.gr[GRAD ? t : 0][0] = at ? lx : L.gr[GRAD ? t : 0][0];
+            L.gr[GRAD ? t : 0][1] = at ? ly : L.gr[GRAD ? t : 0][1];
+            L.gr[GRAD ? t : 0][2] = at ? lz : L.gr[GRAD ? t : 0][2];
         }
     }
-    bool wide = n0 > 1;
-#pragma unroll
-    for (int k = 0; k < 27; ++k)
-        wide = wide || ListCode<TV>::count(static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)))) > 1;
-    if (!wide) return kCellBulk;
+    ne += app ? 1 : 0;
+}
 
-    // Merge plane by plane (ii = -1, 0, 1): the first two entries of the plane's nine neighbours
-    // are loaded together (branch-free buffer loads), later entries (rare) one by one; entries
-    // are merged in stencil order.
+// Merges the neighbour lists of a wide cell into L in stencil order, accumulating Laplacians (and
+// gradients); appended entries start at value 0 (CalculateDerivativesSR). One plane of nine
+// neighbours per loop trip (the loop is not unrolled, which keeps the kernel within the
+// instruction cache): the first two entries of the nine are loaded together with branch-free
+// buffer loads, a third entry (rare) on demand. A list growing beyond K returns kCellDeferred.
+// Entries beyond a neighbour's third are read one by one (MANY) or defer the cell.
+template <int K, typename TV, typename TC, bool GRAD, bool MANY>
+__device__ __forceinline__ int merge_neighbours(const StepArgs& A, const Box& B, const uint32_t codes[7],
+                                                CellList<K, TC, GRAD>& L)
+{
+    const int64_t N = A.ncells;
+    const uint32_t N32 = static_cast<uint32_t>(N);
+    constexpr uint32_t SV = static_cast<uint32_t>(sizeof(TV));
     const __amdgpu_buffer_rsrc_t ri0 = plane_rsrc(A.id_in, N32 * 2u);
     const __amdgpu_buffer_rsrc_t ri1 = plane_rsrc(A.id_in + N, N32 * 2u);
-    const __amdgpu_buffer_rsrc_t rv0 = plane_rsrc(A.val_in, N32 * static_cast<uint32_t>(sizeof(TV)));
-    const __amdgpu_buffer_rsrc_t rv1 = plane_rsrc(static_cast<const TV*>(A.val_in) + N, N32 * static_cast<uint32_t>(sizeof(TV)));
-    int ne = n0;
+    const __amdgpu_buffer_rsrc_t rv0 = plane_rsrc(A.val_in, N32 * SV);
+    const __amdgpu_buffer_rsrc_t rv1 = plane_rsrc(static_cast<const TV*>(A.val_in) + N, N32 * SV);
+    int ne = L.n0;
+    bool over = false;
+    const int ip_end = A.dnx ? 3 : 2;
+#pragma unroll 1
+    for (int ip = A.dnx ? 0 : 1; ip < ip_end; ++ip) {
+        // the plane's count bytes and row starts (selects: ip is a loop variable)
+        uint8_t pc[9];
 #pragma unroll
-    for (int ip = 0; ip < 3; ++ip) {
-        if (ip != 1 && !A.dnx) continue;
+        for (int j = 0; j < 9; ++j)
+            pc[j] = ip == 0 ? box_code(codes, j) : (ip == 1 ? box_code(codes, 9 + j) : box_code(codes, 18 + j));
+        uint32_t prow[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) prow[r] = ip == 0 ? B.rows[r] : (ip == 1 ? B.rows[3 + r] : B.rows[6 + r]);
         TV v0[9], v1[9];
         uint32_t g0[9], g1[9];
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
-            const int k = 9 * ip + j;
-            const uint8_t b = static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)));
-            const int m = ListCode<TV>::count(b), imp = ListCode<TV>::implicit(b);
-            const uint32_t nb = neighbour(k);
+            const int m = ListCode<TV>::count(pc[j]), imp = ListCode<TV>::implicit(pc[j]);
+            const uint32_t nb = prow[j / 3] + B.zs[j % 3];
             g0[j] = buffer_load<uint16_t>(ri0, m > 0 ? nb * 2u : kNoLoad);
-            v0[j] = buffer_load<TV>(rv0, m > 0 && imp != 0 ? nb * static_cast<uint32_t>(sizeof(TV)) : kNoLoad);
+            v0[j] = buffer_load<TV>(rv0, m > 0 && imp != 0 ? nb * SV : kNoLoad);
             g1[j] = buffer_load<uint16_t>(ri1, m > 1 ? nb * 2u : kNoLoad);
-            v1[j] = buffer_load<TV>(rv1, m > 1 && imp != 1 ? nb * static_cast<uint32_t>(sizeof(TV)) : kNoLoad);
+            v1[j] = buffer_load<TV>(rv1, m > 1 && imp != 1 ? nb * SV : kNoLoad);
         }
+        const int dx = ip - 1;
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
-            const int k = 9 * ip + j;
-            const int ii = ip - 1, jj = j / 3 - 1, kk = j % 3 - 1;
-            const int64_t nb = neighbour(k);
-            const int nnz = (ii != 0) + (jj != 0) + (kk != 0);
-            const TC w = static_cast<TC>(A.wl[nnz]);
-            const TC gw = static_cast<TC>(A.wg[nnz]);
-            const TC gwx = ii == 0 ? TC(0) : (ii < 0 ? -gw : gw);
+            const int jj = j / 3 - 1, kk = j % 3 - 1;
+            const int m = ListCode<TV>::count(pc[j]), imp = ListCode<TV>::implicit(pc[j]);
+            const uint32_t nb = prow[j / 3] + B.zs[j % 3];
+            const int nj = (jj != 0) + (kk != 0);
+            const bool side = dx != 0 || nj != 0;
+            const TC w = static_cast<TC>(dx != 0 ? A.wl[nj + 1] : A.wl[nj]);
+            const TC gw = static_cast<TC>(dx != 0 ? A.wg[nj + 1] : A.wg[nj]);
+            const TC gwx = dx == 0 ? TC(0) : (dx < 0 ? -gw : gw);
             const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
             const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
-            const uint8_t nbcode = static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)));
-            const int m = ListCode<TV>::count(nbcode);
-            const int imp = ListCode<TV>::implicit(nbcode);
+            // third entry (rare) before anything else: the implicit value may need it
+            TV v2 = TV(0);
+            uint32_t g2 = 0u;
+            if (m > 2) {
+                g2 = A.id_in[2 * N + nb];
+                if (imp != 2) v2 = load_val<TV>(A.val_in, 2 * N + nb);
+            }
             // implicit entry: 1 - sum of the others in slot order
             double vimp = 0.0;
-            if (ListCode<TV>::kImplicit && imp >= 0 && m > 0) {
-                if (m == 1) vimp = 1.0;
-                else if (m == 2) vimp = 1.0 - (0.0 + static_cast<double>(imp == 0 ? v1[j] : v0[j]));
-                else vimp = implicit_value<TV>(A, nb, m, imp);
-            }
-            for (int q = 0; q < m; ++q) {
-                TC gv;
-                uint32_t gid;
-                if (ListCode<TV>::kImplicit && q == imp) gv = static_cast<TC>(vimp);
-                else if (q == 0) gv = static_cast<TC>(v0[j]);
-                else if (q == 1) gv = static_cast<TC>(v1[j]);
-                else gv = static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
-                if (gv == TC(0)) continue;
-                if (q == 0) gid = g0[j];
-                else if (q == 1) gid = g1[j];
-                else gid = A.id_in[static_cast<int64_t>(q) * N + nb];
-                const TC l = w * gv;
-                const TC lx = GRAD && nnz ? gwx * gv : TC(0);
-                const TC ly = GRAD && nnz ? gwy * gv : TC(0);
-                const TC lz = GRAD && nnz ? gwz * gv : TC(0);
-                bool found = false;
-                // branch-free: adding +0 leaves a sum unchanged (the sums are never -0)
-#pragma unroll U
-                for (int t = 0; t < K; ++t) {
-                    const bool mt = t < ne && L.eid[t] == gid;
-                    L.lap[t] += mt ? l : TC(0);
-                    if (GRAD && nnz) {
-                        L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][0] + (mt ? lx : TC(0));
-                        L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][1] + (mt ? ly : TC(0));
-                        L.gr[GRAD ? t : 0][2] = L.gr[GRAD ? t : 0][2] + (mt ? lz : TC(0));
-                    }
-                    found = found || mt;
+            if (ListCode<TV>::kImplicit) {
+                if (m > 3) {
+                    vimp = implicit_value<TV>(A, nb, m, imp);
+                } else {
+                    double sum = 0.0;
+                    if (m > 0 && imp != 0) sum += static_cast<double>(v0[j]);
+                    if (m > 1 && imp != 1) sum += static_cast<double>(v1[j]);
+                    if (m > 2 && imp != 2) sum += static_cast<double>(v2);
+                    vimp = 1.0 - sum;
                 }
-                if (!found) {
-                    if (ne >= K) return kCellDeferred;
-#pragma unroll U
-                    for (int t = 0; t < K; ++t)
-                        if (t == ne) {
-                            L.eid[t] = gid;
-                            L.lap[t] = l;
-                            if (GRAD) {
-                                L.gr[GRAD ? t : 0][0] = lx;
-                                L.gr[GRAD ? t : 0][1] = ly;
-                                L.gr[GRAD ? t : 0][2] = lz;
-                            }
-                        }
-                    ++ne;
+            }
+            auto value = [&](int q, TV raw) -> TC {
+                return (ListCode<TV>::kImplicit && q == imp) ? static_cast<TC>(vimp) : static_cast<TC>(raw);
+            };
+            const TC a0 = value(0, v0[j]), a1 = value(1, v1[j]);
+            merge_entry<K, TC, GRAD>(L, ne, over, m > 0 && a0 != TC(0), g0[j], w * a0,
+                                     gwx * a0, gwy * a0, gwz * a0, side);
+            merge_entry<K, TC, GRAD>(L, ne, over, m > 1 && a1 != TC(0), g1[j], w * a1,
+                                     gwx * a1, gwy * a1, gwz * a1, side);
+            if (m > 2) {
+                const TC a2 = value(2, v2);
+                merge_entry<K, TC, GRAD>(L, ne, over, a2 != TC(0), g2, w * a2, gwx * a2, gwy * a2, gwz * a2, side);
+            }
+            if (m > 3) {
+                if (!MANY) {
+                    over = true;
+                } else {
+                    for (int q = 3; q < m; ++q) {
+                        const TC aq = value(q, load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
+                        const uint32_t gq = A.id_in[static_cast<int64_t>(q) * N + nb];
+                        merge_entry<K, TC, GRAD>(L, ne, over, aq != TC(0), gq, w * aq, gwx * aq, gwy * aq, gwz * aq, side);
+                    }
                 }
             }
         }
     }
     L.ne = ne;
-    return kCellWide;
+    return over ? kCellDeferred : kCellWide;
+}
+
+// Own list plus neighbour merge of cell cc (the work-list kernels). Returns kCellBulk (nothing to
+// do), kCellWide, or kCellDeferred when more than K entries are needed.
+template <int K, typename TV, typename TC, bool GRAD>
+__device__ __forceinline__ int gather_cell(const StepArgs& A, int64_t cc, CellList<K, TC, GRAD>& L)
+{
+    const uint8_t code = A.cnt_in[cc];
+    if (load_own<K, TV, TC, GRAD>(A, cc, code, L) == kCellDeferred) return kCellDeferred;
+    const Box B = make_box(A, cc);
+    uint32_t codes[7];
+    if (!load_codes<TV>(A, B, code, codes)) return kCellBulk;
+    return merge_neighbours<K, TV, TC, GRAD, true>(A, B, codes, L);
 }
 
 // IP.Set + increments + normalize + merge + finalize of a wide cell whose merged list has C
 // entries (EXACT) or at most C entries (ne at run time). Only entries < C are touched.
-template <int C, bool EXACT, int K, typename TC, bool GRAD, bool DG>
-__device__ __forceinline__ void compute_cell(const StepArgs& A, CellList<K, TC, GRAD>& L, bool& empty)
+// PATH selects the increment formula at compile time: 0 both (chosen per cell), 1 only the
+// equal-energy form (returns false, doing nothing, for a cell that needs the general one), 2 only
+// the general form (exact for every cell: the equal-energy form drops exact zero terms).
+template <int C, bool EXACT, int K, typename TC, bool GRAD, bool DG, int PATH = 0>
+__device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, GRAD>& L, bool& empty)
 {
     constexpr int U = C <= kFastSlots ? C : 1;
     constexpr int NPC = C * (C - 1) / 2;
@@ -422,6 +496,8 @@ __device__ __forceinline__ void compute_cell(const StepArgs& A, CellList<K, TC, 
     bool uni = A.uniform != 0;
 #pragma unroll U
     for (int t = 0; t < C; ++t) uni = uni && (!(EXACT || t < ne) || vr[t] == 1.0);
+    if (PATH == 1 && !uni) return false;
+    if (PATH == 2) uni = false;
 
     // ---- double-obstacle pair increments
     const TC P = static_cast<TC>(A.prefactor);
@@ -701,6 +777,7 @@ __device__ __forceinline__ void compute_cell(const StepArgs& A, CellList<K, TC, 
     }
 #pragma unroll
     for (int t = C; t < K; ++t) L.keep[t] = false;
+    return true;
 }
 
 // Writes the finalized list of cell cc (compacted, order preserved); rounds phi to the storage
@@ -765,42 +842,142 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
-template <typename TV, typename TC, bool GRAD, bool DG>
-__global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs A)
+// wave_volume_add for entries of exactly 1.0 (single-entry cells, most of the grid) by integer
+// popcounts: 1.0 is 2^116 in the fixed-point sums, bit 20 of limb 3. Other values take the
+// FP64 butterfly. Must be reached by whole wavefronts.
+__device__ __forceinline__ void wave_volume_add_split(bool act, uint32_t g, double v,
+                                                      unsigned long long* acc, bool& bad)
+{
+    const int lane = static_cast<int>(threadIdx.x) & (warpSize - 1);
+    bool one = act && v == 1.0;
+    while (true) {
+        const unsigned long long m = __ballot(one);
+        if (m == 0ull) break;
+        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+        const uint32_t g0 = static_cast<uint32_t>(__shfl(static_cast<int>(g), leader));
+        const bool sel = one && g == g0;
+        const unsigned long long cells = static_cast<unsigned long long>(__popcll(__ballot(sel)));
+        if (lane == leader) atomicAdd(&acc[static_cast<int64_t>(g0) * kLimbs + 3], cells << 20);
+        one = one && !sel;
+    }
+    wave_volume_add<false>(act && v != 1.0, g, v, acc, bad);
+}
+
+// One block = kBlock consecutive cells. Phase 1 classifies every cell from the count bytes of its
+// box: bulk cells (one entry, no multi-entry neighbour) keep their list and are copied through;
+// wide cells are compacted in thread order into LDS. Phase 2 runs full wavefronts of wide cells
+// through merge, pair increments, limiter, merge and finalize, so bulk lanes no longer ride along
+// with interface lanes. Cells needing more than kMainSlots entries go to the mid kernel's list.
+template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
+__global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A)
 {
     constexpr int K = kMainSlots;
-    const int64_t c = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+    constexpr int kWaves = kBlock / 64;
+    __shared__ uint32_t s_codes[7][kBlock];
+    __shared__ uint16_t s_list[kBlock];
+    __shared__ uint32_t s_wide[kWaves];
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int64_t base = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBlock;
+    const int64_t c = base + tid;
     const bool valid = c < A.ncells;
-    CellList<K, TC, GRAD> L;
-    bool empty = false, bad_volume = false;
-    int r = kCellBulk;
+    bool bad_volume = false;
+
+    // ---- phase 1: classify; copy bulk cells
+    bool wide = false, single = false, defer = false;
+    uint32_t g = 0u;
+    double v = 0.0;
     if (valid) {
-        r = gather_cell<K, TV, TC, GRAD>(A, c, L);
-        if (r == kCellWide) {
-            // merged lists of 2-4 entries here; the rest (few cells) keep this kernel's register
-            // budget low by going to the mid kernel
-            switch (L.ne) {
-            case 2: compute_cell<2, true, K, TC, GRAD, DG>(A, L, empty); break;
-            case 3: compute_cell<3, true, K, TC, GRAD, DG>(A, L, empty); break;
-            case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty); break;
-            default: r = kCellDeferred; break;
+        const uint8_t code = A.cnt_in[c];
+        const int n0 = ListCode<TV>::count(code);
+        const Box B = make_box(A, c);
+        uint32_t codes[7];
+        if (load_codes<TV>(A, B, code, codes)) {
+            if (n0 > K) {
+                defer = true;
+            } else {
+                wide = true;
+#pragma unroll
+                for (int w = 0; w < 7; ++w) s_codes[w][tid] = codes[w];
             }
+        } else if (n0 == 1) {
+            // NodePF::finalize keeps a lone entry at 1; an FP32 list stores it implicitly
+            single = true;
+            g = A.id_in[c];
+            A.id_out[c] = static_cast<uint16_t>(g);
+            if (ListCode<TV>::kImplicit) {
+                v = 1.0;
+                A.cnt_out[c] = 1;
+            } else {
+                v = static_cast<double>(load_val<TV>(A.val_in, c));
+                static_cast<TV*>(A.val_out)[c] = static_cast<TV>(v);
+                A.cnt_out[c] = code;
+            }
+        } else {
+            A.cnt_out[c] = 0;  // empty node (reported when a step empties it)
+        }
+    }
+    if (defer) {
+        const unsigned slot = atomicAdd(&A.work_count[0], 1u);
+        A.work_list[slot] = static_cast<uint32_t>(c);
+    }
+    wave_volume_add_split(single, g, v, A.vol_acc, bad_volume);
+
+    // ---- compaction of the wide cells in thread order (deterministic)
+    const unsigned long long wm = __ballot(wide);
+    if (lane == 0) s_wide[wave] = static_cast<uint32_t>(__popcll(wm));
+    __syncthreads();
+    int offset = __popcll(wm & ((1ull << lane) - 1ull)), nwide = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int n = static_cast<int>(s_wide[w]);
+        offset += w < wave ? n : 0;
+        nwide += n;
+    }
+    if (wide) s_list[offset] = static_cast<uint16_t>(tid);
+    __syncthreads();
+    if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
+    bad_volume = false;
+    if (wave * 64 >= nwide) return;  // whole wavefront; no barrier follows
+
+    // ---- phase 2: wide cells
+    CellList<K, TC, GRAD> L;
+#pragma unroll
+    for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
+    if (tid < nwide) {
+        const int local = s_list[tid];
+        const int64_t cw = base + local;
+        uint32_t codes[7];
+#pragma unroll
+        for (int w = 0; w < 7; ++w) codes[w] = s_codes[w][local];
+        bool empty = false;
+        load_own<K, TV, TC, GRAD>(A, cw, box_code(codes, 13), L);
+        int r = merge_neighbours<K, TV, TC, GRAD, true>(A, make_box(A, cw), codes, L);
+        if (r == kCellWide) {
+            // merged lists of 2-4 entries here; longer ones (few cells) go to the mid kernel
+            // UNI: the equal-energy form only; cells next to a grain below its reference volume
+            // (VolumeRatio < 1) go to the mid kernel
+            constexpr int PATH = UNI ? 1 : 2;
+            bool done = false;
+            switch (L.ne) {
+            case 2: done = compute_cell<2, true, K, TC, GRAD, DG, PATH>(A, L, empty); break;
+            case 3: done = compute_cell<3, true, K, TC, GRAD, DG, PATH>(A, L, empty); break;
+            case 4: done = compute_cell<4, true, K, TC, GRAD, DG, PATH>(A, L, empty); break;
+            default: break;
+            }
+            if (!done) r = kCellDeferred;
         }
         if (r == kCellDeferred) {
 #pragma unroll
             for (int t = 0; t < K; ++t) L.keep[t] = false;
             const unsigned slot = atomicAdd(&A.work_count[0], 1u);
-            A.work_list[slot] = static_cast<uint32_t>(c);
-        } else if (!store_cell<K, TV, TC, GRAD>(A, c, L)) {
-            report_overflow<K, TC, GRAD>(A, c, L);
+            A.work_list[slot] = static_cast<uint32_t>(cw);
+        } else if (!store_cell<K, TV, TC, GRAD>(A, cw, L)) {
+            report_overflow<K, TC, GRAD>(A, cw, L);
         }
         if (empty) atomicOr(&A.status[kStatusEmpty], 1);
-    } else {
-#pragma unroll
-        for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
     }
 #pragma unroll
-    for (int t = 0; t < K; ++t) wave_volume_add<false>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
+    for (int t = 0; t < K; ++t) wave_volume_add_split(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
     if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
 }
 
@@ -823,8 +1000,12 @@ __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs A)
             const int64_t c = A.work_list[i];
             int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
             if (r == kCellWide) {
-                // 5 and 6 entries in registers; 1, 7, 8 (rare) with the 16-entry path
+                // 2-6 entries in registers (the main kernel passes on 2-4 entry cells next to a
+                // grain below its reference volume); 1, 7, 8 (rare) with the 16-entry path
                 switch (L.ne) {
+                case 2: compute_cell<2, true, K, TC, GRAD, DG>(A, L, empty); break;
+                case 3: compute_cell<3, true, K, TC, GRAD, DG>(A, L, empty); break;
+                case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty); break;
                 case 5: compute_cell<5, true, K, TC, GRAD, DG>(A, L, empty); break;
                 case 6: compute_cell<6, true, K, TC, GRAD, DG>(A, L, empty); break;
                 default: r = kCellDeferred; break;
@@ -1025,12 +1206,15 @@ int launch_step(const StepArgs& a, bool fp32, bool full, void* stream)
 {
     const dim3 grid(static_cast<unsigned>((a.ncells + kBlock - 1) / kBlock));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool uni = a.uniform != 0;
     if (fp32) {
-        if (full) hipLaunchKernelGGL((step_kernel<float, float, true, true>), grid, dim3(kBlock), 0, s, a);
-        else      hipLaunchKernelGGL((step_kernel<float, float, false, false>), grid, dim3(kBlock), 0, s, a);
+        if (full)     hipLaunchKernelGGL((step_kernel<float, float, true, true, false>), grid, dim3(kBlock), 0, s, a);
+        else if (uni) hipLaunchKernelGGL((step_kernel<float, float, false, false, true>), grid, dim3(kBlock), 0, s, a);
+        else          hipLaunchKernelGGL((step_kernel<float, float, false, false, false>), grid, dim3(kBlock), 0, s, a);
     } else {
-        if (full) hipLaunchKernelGGL((step_kernel<double, double, true, true>), grid, dim3(kBlock), 0, s, a);
-        else      hipLaunchKernelGGL((step_kernel<double, double, false, false>), grid, dim3(kBlock), 0, s, a);
+        if (full)     hipLaunchKernelGGL((step_kernel<double, double, true, true, false>), grid, dim3(kBlock), 0, s, a);
+        else if (uni) hipLaunchKernelGGL((step_kernel<double, double, false, false, true>), grid, dim3(kBlock), 0, s, a);
+        else          hipLaunchKernelGGL((step_kernel<double, double, false, false, false>), grid, dim3(kBlock), 0, s, a);
     }
     return static_cast<int>(hipGetLastError());
 }
