@@ -165,6 +165,31 @@ mpf_status mpf_step_stats(mpf_ctx* ctx, int64_t* deferred_mid, int64_t* deferred
  * calibration kernel for rocprof FETCH_SIZE/WRITE_SIZE. */
 mpf_status mpf_copy_bandwidth(mpf_ctx* ctx, int64_t bytes, int32_t repeats, double* ms);
 
+/* Replaces the phase-pair tables (nphases x nphases, row-major; any pointer may be NULL to keep
+ * that table): interface energy (InterfaceProperties' Energy / EnergyModel, src/InterfaceProperties.cpp
+ * :229-291), mobility (MaxMobility), constant driving force (UserDrivingForce Value,
+ * src/UserDrivingForce.cpp:60-98; the context must have been created with one). MaxEnergy and the
+ * driving-force limiter amplitude are rederived as at creation. For calibration loops. */
+mpf_status mpf_set_pair_params(mpf_ctx* ctx, const double* sigma, const double* mu, const double* dg_value);
+
+/* Checkpoints (no reference counterpart; segment checkpointing of adjoint rollouts): the size of
+ * a saved state, and save / restore of the current state - lists and grain volumes, MAXVolume,
+ * VolumeRatio, Exist, Stage - to / from caller-owned DEVICE memory of that size. */
+mpf_status mpf_state_bytes(mpf_ctx* ctx, int64_t* bytes);
+mpf_status mpf_save_state(mpf_ctx* ctx, void* device_dst);
+mpf_status mpf_restore_state(mpf_ctx* ctx, const void* device_src);
+
+/* Adjoint of one FP64 step from the current state S_t with time step dt (no reference
+ * counterpart: the reference has no derivatives). All pointers are DEVICE memory:
+ *   lam_next  [slots * ncells]  cotangent of the values of S_{t+1}, slot-major like the state;
+ *   lam_prev  [slots * ncells]  written: cotangent of the values of S_t (0 beyond each count);
+ *   grad      [3 * nphases^2]   accumulated: cotangents of the sigma, mu and driving-force
+ *                               tables (entries as read, before any symmetrization).
+ * Derivatives follow the branches the forward step takes (clamps, erasures, limiter factors);
+ * VolumeRatios are held fixed. Requires FP64, at most 4 phases and merged lists of at most 8
+ * entries (MPF_ERR_UNSUPPORTED otherwise). */
+mpf_status mpf_adjoint_step(mpf_ctx* ctx, double dt, const double* lam_next, double* lam_prev, double* grad);
+
 /* Human-readable description of the last error on this context (static storage per context). */
 const char* mpf_error_message(mpf_ctx* ctx);
 

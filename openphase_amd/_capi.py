@@ -26,7 +26,8 @@ EXPORTED = [
     "mpf_create", "mpf_set_grains", "mpf_upload", "mpf_finalize_initialization", "mpf_step",
     "mpf_step_async", "mpf_synchronize", "mpf_total_entries", "mpf_download", "mpf_get_grains",
     "mpf_max_time_step", "mpf_limiter_warnings", "mpf_overflow_cell", "mpf_device_state",
-    "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_error_message",
+    "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
+    "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_error_message",
     "mpf_destroy",
 ]
 
@@ -92,6 +93,11 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_enable_timing": ([vp, C.c_int32], C.c_int),
         "mpf_copy_bandwidth": ([vp, C.c_int64, C.c_int32, f64p], C.c_int),
         "mpf_step_stats": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+        "mpf_set_pair_params": ([vp, f64p, f64p, f64p], C.c_int),
+        "mpf_state_bytes": ([vp, C.POINTER(C.c_int64)], C.c_int),
+        "mpf_save_state": ([vp, vp], C.c_int),
+        "mpf_restore_state": ([vp, vp], C.c_int),
+        "mpf_adjoint_step": ([vp, C.c_double, vp, vp, vp], C.c_int),
         "mpf_error_message": ([vp], C.c_char_p),
         "mpf_destroy": ([vp], None),
     }

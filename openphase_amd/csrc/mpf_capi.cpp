@@ -55,7 +55,7 @@ struct mpf_ctx {
     bool uniform = false;            // see StepArgs::uniform
     int tj_model = 0;
     int consider_nucleus = 1;
-    double nucleus_volume_factor = 1.0, regularization = 1.0;
+    double nucleus_volume_factor = 1.0, regularization = 1.0, dg_limit = 0.0;
     double wl[4] = {0, 0, 0, 0}, wg[4] = {0, 0, 0, 0};
     double prefactor = 0.0, dg_prefactor = 0.0;
 
@@ -85,6 +85,13 @@ struct mpf_ctx {
     int32_t* d_dgmode = nullptr;
     double* d_dgval = nullptr;
     double* d_dgallowed = nullptr;
+
+    // adjoint work arrays (allocated on first use)
+    double* a_lam_own = nullptr;
+    uint8_t* a_cnt = nullptr;
+    uint16_t* a_id = nullptr;
+    double* a_lap = nullptr;
+    double* a_grad = nullptr;
 
     int32_t* d_status = nullptr;
     unsigned long long* d_first_overflow = nullptr;
@@ -256,6 +263,7 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     c->dg_prefactor = kPi / eta;
 
     c->dg = params->dg_mode != nullptr && params->dg_value != nullptr;
+    c->dg_limit = params->dg_limit;
     c->dgmode.assign(np2, 0);
     c->dgval.assign(np2, 0.0);
     c->dgallowed.assign(np2, 1.0);
@@ -691,12 +699,164 @@ void mpf_destroy(mpf_ctx* c)
     for (auto e : c->events) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b) { dfree(c->cnt[b]); dfree(c->id[b]); dfree(c->val[b]); }
     free_grains(c);
+    dfree(c->a_lam_own); dfree(c->a_cnt); dfree(c->a_id); dfree(c->a_lap); dfree(c->a_grad);
     dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
     dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+// ------------------------------------------------------------------ parameters, checkpoints, adjoint
+
+mpf_status mpf_set_pair_params(mpf_ctx* c, const double* sigma, const double* mu, const double* dg_value)
+{
+    if (!c) return MPF_ERR_ARG;
+    const size_t np2 = static_cast<size_t>(c->nphases) * c->nphases;
+    if (dg_value && !c->dg) return c->fail(MPF_ERR_ARG, "the context was created without a driving force");
+    if (sigma) c->sigma.assign(sigma, sigma + np2);
+    if (mu) c->mu.assign(mu, mu + np2);
+    if (dg_value) c->dgval.assign(dg_value, dg_value + np2);
+    // derived tables as in mpf_create: MaxEnergy, the tanh amplitude, the equal-energy flag
+    for (size_t i = 0; i < np2; ++i)
+        c->maxe[i] = c->emodel[i] == MPF_ENERGY_CUBIC ? c->sigma[i] * (1.0 + c->eps[i] * 1.41) : c->sigma[i];
+    c->uniform = !c->grad && ((c->tj_model == MPF_TJ_MODEL && c->tj_factor == 0.0) || c->tj_model == MPF_TJ_NONE);
+    for (size_t i = 0; i < np2; ++i)
+        if (c->sigma[i] != c->sigma[0] || c->mu[i] != c->mu[0]) c->uniform = false;
+    if (c->dg)
+        for (size_t k = 0; k < np2; ++k) {
+            double allowed = c->dg_limit * c->dg_prefactor * c->maxe[k];
+            allowed *= c->regularization;
+            c->dgallowed[k] = allowed;
+        }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    h2d(e, c->d_sigma, c->sigma.data(), np2);
+    h2d(e, c->d_mu, c->mu.data(), np2);
+    h2d(e, c->d_maxe, c->maxe.data(), np2);
+    h2d(e, c->d_dgval, c->dgval.data(), np2);
+    h2d(e, c->d_dgallowed, c->dgallowed.data(), np2);
+    return c->hip(e, "pair tables");
+}
+
+namespace {
+
+// Byte offsets of the parts of a saved state: count, index, value planes, then the grain arrays
+// (volume, MAXVolume, VolumeRatio, Exist, Stage), each part 256-byte aligned.
+struct StateLayout {
+    size_t cnt, id, val, volume, maxvolume, vr, exist, stage, total;
+};
+
+StateLayout state_layout(const mpf_ctx* c)
+{
+    auto up = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+    const size_t N = static_cast<size_t>(c->ncells), S = static_cast<size_t>(c->slots), G = static_cast<size_t>(c->ngrains);
+    StateLayout L{};
+    L.cnt = 0;
+    L.id = up(L.cnt + N);
+    L.val = up(L.id + S * N * sizeof(uint16_t));
+    L.volume = up(L.val + S * N * c->vsize());
+    L.maxvolume = up(L.volume + G * sizeof(double));
+    L.vr = up(L.maxvolume + G * sizeof(double));
+    L.exist = up(L.vr + G * sizeof(double));
+    L.stage = up(L.exist + G);
+    L.total = up(L.stage + G * sizeof(int32_t));
+    return L;
+}
+
+}  // namespace
+
+mpf_status mpf_state_bytes(mpf_ctx* c, int64_t* bytes)
+{
+    if (!c || !bytes) return MPF_ERR_ARG;
+    if (!c->have_grains) return c->fail(MPF_ERR_STATE, "mpf_set_grains first");
+    *bytes = static_cast<int64_t>(state_layout(c).total);
+    return MPF_OK;
+}
+
+mpf_status mpf_save_state(mpf_ctx* c, void* dst)
+{
+    if (!c || !dst) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
+    const StateLayout L = state_layout(c);
+    const size_t N = static_cast<size_t>(c->ncells), S = static_cast<size_t>(c->slots), G = static_cast<size_t>(c->ngrains);
+    char* d = static_cast<char*>(dst);
+    hipError_t e = hipSuccess;
+    auto cp = [&](size_t off, const void* src, size_t n) {
+        if (e == hipSuccess) e = hipMemcpyAsync(d + off, src, n, hipMemcpyDeviceToDevice, c->stream);
+    };
+    cp(L.cnt, c->cnt[c->cur], N);
+    cp(L.id, c->id[c->cur], S * N * sizeof(uint16_t));
+    cp(L.val, c->val[c->cur], S * N * c->vsize());
+    cp(L.volume, c->g_volume, G * sizeof(double));
+    cp(L.maxvolume, c->g_maxvolume, G * sizeof(double));
+    cp(L.vr, c->g_vr, G * sizeof(double));
+    cp(L.exist, c->g_exist, G);
+    cp(L.stage, c->g_stage, G * sizeof(int32_t));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return c->hip(e, "state save");
+}
+
+mpf_status mpf_restore_state(mpf_ctx* c, const void* src)
+{
+    if (!c || !src) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
+    const StateLayout L = state_layout(c);
+    const size_t N = static_cast<size_t>(c->ncells), S = static_cast<size_t>(c->slots), G = static_cast<size_t>(c->ngrains);
+    const char* s = static_cast<const char*>(src);
+    hipError_t e = hipSuccess;
+    auto cp = [&](void* dstp, size_t off, size_t n) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dstp, s + off, n, hipMemcpyDeviceToDevice, c->stream);
+    };
+    cp(c->cnt[c->cur], L.cnt, N);
+    cp(c->id[c->cur], L.id, S * N * sizeof(uint16_t));
+    cp(c->val[c->cur], L.val, S * N * c->vsize());
+    cp(c->g_volume, L.volume, G * sizeof(double));
+    cp(c->g_maxvolume, L.maxvolume, G * sizeof(double));
+    cp(c->g_vr, L.vr, G * sizeof(double));
+    cp(c->g_exist, L.exist, G);
+    cp(c->g_stage, L.stage, G * sizeof(int32_t));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return c->hip(e, "state restore");
+}
+
+mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, double* lam_prev, double* grad)
+{
+    if (!c || !lam_next || !lam_prev || !grad) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
+    if (c->fp32) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint runs on FP64 states");
+    if (c->nphases > 4) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint handles at most 4 phases");
+    const size_t N = static_cast<size_t>(c->ncells);
+    constexpr size_t KA = mpf::kAdjSlots;
+    if (!c->a_cnt) {
+        if (c->hip(dalloc(&c->a_lam_own, KA * N), "adjoint arrays") || c->hip(dalloc(&c->a_cnt, N), "adjoint arrays") ||
+            c->hip(dalloc(&c->a_id, KA * N), "adjoint arrays") || c->hip(dalloc(&c->a_lap, KA * N), "adjoint arrays"))
+            return MPF_ERR_NOMEM;
+        if (c->grad && c->hip(dalloc(&c->a_grad, 3 * KA * N), "adjoint arrays")) return MPF_ERR_NOMEM;
+        if (c->hip(hipMemset(c->a_cnt, 0, N), "adjoint arrays")) return MPF_ERR_HIP;
+    }
+    mpf::AdjointArgs a{};
+    a.s = step_args(c, dt, c->cur);
+    a.lam_next = lam_next;
+    a.lam_prev = lam_prev;
+    a.lam_own = c->a_lam_own;
+    a.adj_cnt = c->a_cnt;
+    a.adj_id = c->a_id;
+    a.adj_lap = c->a_lap;
+    a.adj_grad = c->a_grad;
+    a.grad = grad;
+    a.dg_limit = c->dg_limit;
+    a.regularization = c->regularization;
+    a.grad_energy = c->grad ? 1 : 0;
+    a.driving_force = c->dg ? 1 : 0;
+    const int32_t zeros[mpf::kStatusWords] = {0, 0, 0, 0};
+    mpf_status st = c->hip(hipMemcpyAsync(c->d_status, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream), "status reset");
+    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_adjoint(a, c->stream)), "adjoint kernels");
+    int32_t words[mpf::kStatusWords];
+    if (!st) st = c->hip(hipMemcpyAsync(words, c->d_status, sizeof(words), hipMemcpyDeviceToHost, c->stream), "status readback");
+    if (!st) st = c->hip(hipStreamSynchronize(c->stream), "stream synchronize");
+    if (!st && words[3]) st = c->fail(MPF_ERR_UNSUPPORTED, "a cell's merged list exceeds the adjoint's 8 entries");
+    return st;
 }
 
 }  // extern "C"

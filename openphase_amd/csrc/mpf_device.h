@@ -74,6 +74,26 @@ struct GrainArgs {
     unsigned long long* stats;   // [2] cumulative lengths of the two work lists
 };
 
+// Adjoint of one step (mpf_adjoint.hip). `s` describes the state S_t the step starts from.
+struct AdjointArgs {
+    StepArgs s;
+    const double* lam_next;      // [slots][ncells] cotangent of the values of S_{t+1}
+    double* lam_prev;            // [slots][ncells] cotangent of the values of S_t (written)
+    double* lam_own;             // [kAdjSlots][ncells] direct part per own entry
+    uint8_t* adj_cnt;            // [ncells] merged entries of an interface cell (0: bulk)
+    uint16_t* adj_id;            // [kAdjSlots][ncells]
+    double* adj_lap;             // [kAdjSlots][ncells] cotangent of the Laplacians
+    double* adj_grad;            // [3 kAdjSlots][ncells] cotangent of the gradients (may be null)
+    double* grad;                // [3][nphases^2] cotangents of sigma, mu, driving force (accumulated)
+    double dg_limit;
+    double regularization;
+    int32_t grad_energy;         // orientation-dependent energies present
+    int32_t driving_force;       // constant driving force present
+};
+constexpr int kAdjSlots = 8;     // merged entries per cell the adjoint handles
+
+int launch_adjoint(const AdjointArgs& a, void* stream);
+
 // Host-side launchers (mpf_kernels.hip); return a hipError_t value.
 // `full`: orientation-dependent energies and/or constant driving force present.
 int launch_step(const StepArgs& a, bool fp32, bool full, void* stream);

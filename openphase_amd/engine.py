@@ -210,3 +210,35 @@ class Engine:
         self._check(self.lib.mpf_device_state(self.h, C.byref(cnt), C.byref(idx), C.byref(val), C.byref(stream)),
                     "mpf_device_state")
         return cnt.value, idx.value, val.value, stream.value
+
+    # ---------------------------------------------------------------- calibration / adjoint
+    def set_pair_params(self, sigma=None, mu=None, dg_value=None) -> None:
+        """Replaces the phase-pair tables (nphases x nphases); None keeps a table."""
+        np2 = self.cfg.nphases * self.cfg.nphases
+
+        def tab(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(np2)
+            return a
+        s, m, g = tab(sigma), tab(mu), tab(dg_value)
+        self._keep = (s, m, g)
+        self._check(self.lib.mpf_set_pair_params(self.h, ptr(s, C.c_double), ptr(m, C.c_double), ptr(g, C.c_double)),
+                    "mpf_set_pair_params")
+
+    def state_bytes(self) -> int:
+        n = C.c_int64()
+        self._check(self.lib.mpf_state_bytes(self.h, C.byref(n)), "mpf_state_bytes")
+        return n.value
+
+    def save_state(self, device_ptr: int) -> None:
+        """Copies the current state into device memory of state_bytes() bytes."""
+        self._check(self.lib.mpf_save_state(self.h, C.c_void_p(device_ptr)), "mpf_save_state")
+
+    def restore_state(self, device_ptr: int) -> None:
+        self._check(self.lib.mpf_restore_state(self.h, C.c_void_p(device_ptr)), "mpf_restore_state")
+
+    def adjoint_step(self, dt: float, lam_next: int, lam_prev: int, grad: int) -> None:
+        """One adjoint step from the current state (device pointers, see mpf_adjoint_step)."""
+        self._check(self.lib.mpf_adjoint_step(self.h, float(dt), C.c_void_p(lam_next), C.c_void_p(lam_prev),
+                                              C.c_void_p(grad)), "mpf_adjoint_step")
