@@ -97,6 +97,67 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
     }
 
 
+def run_adjoint(steps: int, segment: int):
+    """Differentiable rollout of the calibration workload: 2D 512^2 periodic, 200 grains of two
+    phases (random half/half), uniform random cubic orientations, cubic anisotropy 0.05,
+    sigma 0.24, mu 4e-9, constant driving force 1e5 J/m^3 between the phases, dt = 0.9 x the 2D
+    limit, FP64. Forward `steps` steps with a checkpoint every `segment` steps, mismatch loss
+    against a target grown with tables perturbed by exp(0.1 N(0,1)), then the adjoint pass
+    (segment recomputation included). Returns (forward + adjoint) steps / wall second."""
+    import torch
+    from openphase_amd import _capi
+    from openphase_amd.adjoint import DeviceStates, dense_fields, mismatch_loss, rollout
+    from openphase_amd.engine import Engine, StepConfig
+    from openphase_amd.microstructure import random_rotations, voronoi_grains
+
+    n, ng = 512, 200
+    cfg = StepConfig(n, n, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                     energy_model=np.full((2, 2), _capi.MPF_ENERGY_CUBIC, np.int32), epsilon=np.full((2, 2), 0.05),
+                     dg_mode=np.array([[0, 1], [1, 0]], np.int32), dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]),
+                     slots=8)
+    dt = 0.9 * cfg.max_time_step()
+    rng = np.random.default_rng(0x0FE4)
+    phase = np.zeros(ng, np.int32)
+    phase[rng.permutation(ng)[: ng // 2]] = 1
+    _, rot = random_rotations(ng, seed=0x0FE4)
+    eng = Engine(cfg)
+    eng.set_grains(phase, rot)
+    eng.upload(voronoi_grains(n, n, 0, ng, seed=0x0FE4))
+    eng.finalize_initialization()
+    st = DeviceStates(eng)
+    init = st.save()
+    p0 = np.stack([cfg.sigma, cfg.mu, cfg.dg_value])
+    f = np.exp(0.1 * np.random.default_rng(0x0FE5).standard_normal(p0.shape))
+    f[0] = np.sqrt(f[0] * f[0].T)
+    f[1] = np.sqrt(f[1] * f[1].T)
+    f[2] = np.triu(f[2]) + np.triu(f[2], 1).T
+    with torch.no_grad():
+        v, c, i = rollout(st, torch.tensor(p0 * f), init, steps, dt, segment=steps)
+        target = dense_fields(v, c, i, ng)
+    params = torch.tensor(p0, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v, c, i = rollout(st, params, init, steps, dt, segment=segment)
+    loss = mismatch_loss(v, c, i, target)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    eng.close()
+    return {
+        "steps_per_s": 2 * steps / (t2 - t0),
+        "forward_steps_per_s": steps / (t1 - t0),
+        "adjoint_steps_per_s": steps / (t2 - t1),
+        "steps": steps,
+        "segment": segment,
+        "loss": float(loss.detach()),
+        "grad_norm": float(params.grad.norm()),
+        "workload": "2D 512^2 periodic, 200 grains, 2 phases, cubic anisotropy 0.05, driving force 1e5 J/m^3, "
+                    "FP64 forward + adjoint with segment checkpointing (steps/s = (forward + adjoint steps) / wall)",
+    }
+
+
 def cpu_baseline(lists, budget_s: float = 30.0):
     """Times the reference OpenPhase build (oracle/_ref/ref_driver, compiled from the reference
     sources) on this host: same 256^3 state, one step, all available cores (max 16)."""
@@ -130,6 +191,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--adjoint-steps", type=int, default=100, help="rollout length of the adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the step on a single MI355X "
@@ -145,6 +207,8 @@ def main():
     print(f"[bench] fp64 {r64}", file=sys.stderr, flush=True)
     r32 = run_gpu(1, lists, args.steps, args.warmup)
     print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
+    adj = run_adjoint(args.adjoint_steps, 25)
+    print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(lists)
     print(f"[bench] cpu {cpu}", file=sys.stderr, flush=True)
 
@@ -192,6 +256,7 @@ def main():
             "fp32_bytes_per_step": r32["bytes_per_step"],
             "copy_kernel_gbs": r64["copy_gbs"],
         },
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
