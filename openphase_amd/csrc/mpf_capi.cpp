@@ -314,7 +314,7 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         c->hip(dalloc(&c->d_dgval, np2), "tables") || c->hip(dalloc(&c->d_dgallowed, np2), "tables") ||
         c->hip(dalloc(&c->d_status, mpf::kStatusWords), "status") ||
         c->hip(dalloc(&c->d_first_overflow, 1), "status") || c->hip(dalloc(&c->d_warnings, 1), "status") ||
-        c->hip(dalloc(&c->d_work_list, 2 * N), "work list") || c->hip(dalloc(&c->d_work_count, 2), "work list") ||
+        c->hip(dalloc(&c->d_work_list, 4 * N), "work list") || c->hip(dalloc(&c->d_work_count, 4), "work list") ||
         c->hip(dalloc(&c->d_stats, 2), "statistics"))
         return bail(MPF_ERR_NOMEM, c->err.c_str());
     hipError_t e = hipSuccess;
@@ -331,8 +331,8 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     h2d(e, c->d_status, zeros, mpf::kStatusWords);
     h2d(e, c->d_first_overflow, &none, 1);
     h2d(e, c->d_warnings, &zero, 1);
-    const unsigned zero_u[2] = {0u, 0u};
-    h2d(e, c->d_work_count, zero_u, 2);
+    const unsigned zero_u[4] = {0u, 0u, 0u, 0u};
+    h2d(e, c->d_work_count, zero_u, 4);
     const unsigned long long zero_s[2] = {0ull, 0ull};
     h2d(e, c->d_stats, zero_s, 2);
     if ((s = c->hip(e, "init copies"))) return bail(s, c->err.c_str());

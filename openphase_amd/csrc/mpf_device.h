@@ -57,8 +57,8 @@ struct StepArgs {
     int32_t* status;             // kStatusWords
     unsigned long long* first_overflow;
     unsigned long long* warnings;
-    uint32_t* work_list;         // deferred cells: [0, ncells) for the mid kernel, [ncells, 2 ncells) for the wide kernel
-    unsigned* work_count;        // [2] lengths of the two lists
+    uint32_t* work_list;         // deferred cells: four lists of ncells each (mid kernel levels 0-2; wide kernel)
+    unsigned* work_count;        // [4] lengths of the four lists
 };
 
 struct GrainArgs {
@@ -70,8 +70,8 @@ struct GrainArgs {
     double* vr;
     uint8_t* exist;
     int32_t* stage;              // 0 seed, 1 nucleus, 2 stable
-    unsigned* work_count;        // [2], reset here for the next step
-    unsigned long long* stats;   // [2] cumulative lengths of the two work lists
+    unsigned* work_count;        // [4], reset here for the next step
+    unsigned long long* stats;   // [2] cumulative lengths of work lists 0 and 1
 };
 
 // Adjoint of one step (mpf_adjoint.hip). `s` describes the state S_t the step starts from.

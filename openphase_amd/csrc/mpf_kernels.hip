@@ -555,13 +555,18 @@ __global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A)
     if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
 }
 
-// Cells the main kernel deferred: merged lists of 5-8 entries (and any other length) in registers;
-// lists longer than 8 go on to the wide kernel.
-template <typename TV, typename TC, bool GRAD, bool DG>
+// Cells the main kernel deferred, with register-resident lists. LEVEL 0 (list 0: merged lists of
+// 5-6 entries, more than 4 own entries, or 2-4 entries next to a grain below its reference
+// volume) computes 2-6 entries with capacity 8; LEVEL 1 (list 1) computes 7 and 8; LEVEL 2
+// (list 2, capacity 12) computes 9 and 10. Anything else moves on to the next list; list 3 belongs
+// to the 16-entry kernel. Splitting the sizes keeps the register count of the common level low
+// and the few long lists off the private-memory path.
+template <typename TV, typename TC, bool GRAD, bool DG, int LEVEL>
 __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs A)
 {
-    constexpr int K = kFastSlots;
-    const unsigned n = A.work_count[0];
+    constexpr int K = LEVEL < 2 ? 8 : 12;
+    const unsigned n = A.work_count[LEVEL];
+    const uint32_t* list = A.work_list + LEVEL * A.ncells;
     const unsigned stride = gridDim.x * kWideBlock;
     for (unsigned base = blockIdx.x * kWideBlock; base < n; base += stride) {
         const unsigned i = base + threadIdx.x;
@@ -571,25 +576,37 @@ __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs A)
 #pragma unroll
         for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
         if (valid) {
-            const int64_t c = A.work_list[i];
+            const int64_t c = list[i];
             int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
             if (r == kCellWide) {
-                // 2-6 entries in registers (the main kernel passes on 2-4 entry cells next to a
-                // grain below its reference volume); 1, 7, 8 (rare) with the 16-entry path
-                switch (L.ne) {
-                case 2: compute_cell<2, true, K, TC, GRAD, DG>(A, L, empty); break;
-                case 3: compute_cell<3, true, K, TC, GRAD, DG>(A, L, empty); break;
-                case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty); break;
-                case 5: compute_cell<5, true, K, TC, GRAD, DG>(A, L, empty); break;
-                case 6: compute_cell<6, true, K, TC, GRAD, DG>(A, L, empty); break;
-                default: r = kCellDeferred; break;
+                if (LEVEL == 0) {
+                    switch (L.ne) {
+                    case 2: compute_cell<2, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 3: compute_cell<3, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 5: compute_cell<5, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 6: compute_cell<6, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    default: r = kCellDeferred; break;
+                    }
+                } else if (LEVEL == 1) {
+                    switch (L.ne) {
+                    case 7: compute_cell<7, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 8: compute_cell<8, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    default: r = kCellDeferred; break;
+                    }
+                } else {
+                    switch (L.ne) {
+                    case 9: compute_cell<9, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 10: compute_cell<10, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    default: r = kCellDeferred; break;
+                    }
                 }
             }
             if (r == kCellDeferred) {
 #pragma unroll
                 for (int t = 0; t < K; ++t) L.keep[t] = false;
-                const unsigned slot = atomicAdd(&A.work_count[1], 1u);
-                A.work_list[A.ncells + slot] = static_cast<uint32_t>(c);
+                const unsigned slot = atomicAdd(&A.work_count[LEVEL + 1], 1u);
+                A.work_list[(LEVEL + 1) * A.ncells + slot] = static_cast<uint32_t>(c);
             } else if (!store_cell<K, TV, TC, GRAD>(A, c, L)) {
                 report_overflow<K, TC, GRAD>(A, c, L);
             }
@@ -606,7 +623,7 @@ template <typename TV, typename TC, bool GRAD, bool DG>
 __global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs A)
 {
     constexpr int K = kWideSlots;
-    const unsigned n = A.work_count[1];
+    const unsigned n = A.work_count[3];
     const unsigned stride = gridDim.x * kWideBlock;
     for (unsigned base = blockIdx.x * kWideBlock; base < n; base += stride) {
         const unsigned i = base + threadIdx.x;
@@ -615,7 +632,7 @@ __global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs A)
         bool empty = false, bad_volume = false;
         for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
         if (valid) {
-            const int64_t c = A.work_list[A.ncells + i];
+            const int64_t c = A.work_list[3 * A.ncells + i];
             const int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
             if (r == kCellWide) compute_cell<K, false, K, TC, GRAD, DG>(A, L, empty);
             if (r == kCellDeferred || !store_cell<K, TV, TC, GRAD>(A, c, L)) report_overflow<K, TC, GRAD>(A, c, L);
@@ -726,6 +743,8 @@ __global__ void grains_kernel(const GrainArgs G)
         G.stats[1] += G.work_count[1];
         G.work_count[0] = 0u;
         G.work_count[1] = 0u;
+        G.work_count[2] = 0u;
+        G.work_count[3] = 0u;
     }
     if (g >= G.ngrains) return;
     unsigned long long l[kLimbs];
@@ -799,18 +818,26 @@ int launch_deferred(const StepArgs& a, bool fp32, bool full, void* stream)
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (fp32) {
         if (full) {
-            hipLaunchKernelGGL((mid_kernel<float, float, true, true>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<float, float, true, true, 0>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<float, float, true, true, 1>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<float, float, true, true, 2>), grid, dim3(kWideBlock), 0, s, a);
             hipLaunchKernelGGL((wide_kernel<float, float, true, true>), grid, dim3(kWideBlock), 0, s, a);
         } else {
-            hipLaunchKernelGGL((mid_kernel<float, float, false, false>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<float, float, false, false, 0>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<float, float, false, false, 1>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<float, float, false, false, 2>), grid, dim3(kWideBlock), 0, s, a);
             hipLaunchKernelGGL((wide_kernel<float, float, false, false>), grid, dim3(kWideBlock), 0, s, a);
         }
     } else {
         if (full) {
-            hipLaunchKernelGGL((mid_kernel<double, double, true, true>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<double, double, true, true, 0>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<double, double, true, true, 1>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<double, double, true, true, 2>), grid, dim3(kWideBlock), 0, s, a);
             hipLaunchKernelGGL((wide_kernel<double, double, true, true>), grid, dim3(kWideBlock), 0, s, a);
         } else {
-            hipLaunchKernelGGL((mid_kernel<double, double, false, false>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<double, double, false, false, 0>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<double, double, false, false, 1>), grid, dim3(kWideBlock), 0, s, a);
+            hipLaunchKernelGGL((mid_kernel<double, double, false, false, 2>), grid, dim3(kWideBlock), 0, s, a);
             hipLaunchKernelGGL((wide_kernel<double, double, false, false>), grid, dim3(kWideBlock), 0, s, a);
         }
     }

@@ -16,7 +16,7 @@ namespace {
 
 
 constexpr int kMainSlots = 4;    // merged-list capacity of the main kernel
-constexpr int kFastSlots = 8;    // merged-list capacity of the register path of the mid kernel
+constexpr int kFastSlots = 12;   // largest capacity compiled with register-resident lists
 constexpr int kWideSlots = 16;   // merged-list capacity of the deferred path
 
 template <int K> __host__ __device__ constexpr int pair_index(int a, int b)
