@@ -378,10 +378,11 @@ __device__ __forceinline__ bool store_cell(const StepArgs& A, int64_t cc, CellLi
     for (int t = 0; t < K; ++t)
         if (L.keep[t]) {
             A.id_out[static_cast<int64_t>(w) * N + cc] = static_cast<uint16_t>(L.eid[t]);
+            const TV sv = static_cast<TV>(L.phi[t]);
             if (IMPL && t == imp_t) {
                 imp_w = w;
+                if (ns > 1) static_cast<TV*>(A.val_out)[static_cast<int64_t>(w) * N + cc] = sv;
             } else {
-                const TV sv = static_cast<TV>(L.phi[t]);
                 L.phi[t] = static_cast<double>(sv);
                 others += L.phi[t];
                 static_cast<TV*>(A.val_out)[static_cast<int64_t>(w) * N + cc] = sv;

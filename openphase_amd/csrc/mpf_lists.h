@@ -161,25 +161,15 @@ __device__ __forceinline__ TC cubic_energy(const double* R, TC gx, TC gy, TC gz,
 enum CellResult { kCellBulk = 0, kCellWide = 1, kCellDeferred = 2, kCellOverflow = 3 };
 
 // Count byte of a stored list. FP64 states store the entry count. FP32 states store
-// count | (implicit slot << 4): the largest entry's value is not stored but recovered as
-// 1 - sum(other entries) in FP64, so values near 1 keep their FP64 resolution and single-entry
-// cells store no value at all.
-// Implicit slot 15 means "none" (every value stored): the form mpf_upload writes.
+// count | (implicit slot << 4): the cell's own update recovers the largest entry as
+// 1 - sum(other entries) in FP64, so values near 1 keep their FP64 resolution; its FP32 value is
+// stored as well for the neighbours' stencils, except in single-entry lists (value 1, nothing
+// stored). Implicit slot 15 means "none" (every value as given): the form mpf_upload writes.
 template <typename TV> struct ListCode {
     static constexpr bool kImplicit = sizeof(TV) == 4;
     __device__ static int count(uint8_t b) { return kImplicit ? (b & 15) : b; }
     __device__ static int implicit(uint8_t b) { return kImplicit && (b >> 4) != 15 ? (b >> 4) : -1; }
 };
-
-// FP64 value of the implicit entry of a stored list of m entries.
-template <typename TV>
-__device__ __forceinline__ double implicit_value(const StepArgs& A, int64_t cell, int m, int imp)
-{
-    double s = 0.0;
-    for (int q = 0; q < m; ++q)
-        if (q != imp) s += static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * A.ncells + cell));
-    return 1.0 - s;
-}
 
 // Per-cell list held in registers (K entries): phase-field index, value (FP64 and in the stencil
 // arithmetic), Laplacian and gradient of the neighbour-merged list.
@@ -291,12 +281,14 @@ __device__ __forceinline__ int load_own(const StepArgs& A, int64_t cc, uint8_t c
 
 // Adds one neighbour entry (grain gid, weighted value l and gradient parts) to the merged list:
 // accumulate into the matching entry or append it (CalculateDerivativesSR appends neighbour
-// indices at value 0). Straight-line selects, no branches; `over` records an append beyond K.
+// indices at value 0). Straight-line selects behind two tests over the active lanes of the
+// wavefront (an entry no lane has, an append no lane needs); `over` records an append beyond K.
 template <int K, typename TC, bool GRAD>
 __device__ __forceinline__ void merge_entry(CellList<K, TC, GRAD>& L, int& ne, bool& over, bool valid,
                                             uint32_t gid, TC l, TC lx, TC ly, TC lz, bool side)
 {
     constexpr int U = K <= kFastSlots ? K : 1;
+    if (!__any(valid)) return;
     bool found = false;
     // adding +0 leaves a sum unchanged (the sums are never -0)
 #pragma unroll U
@@ -311,6 +303,7 @@ __device__ __forceinline__ void merge_entry(CellList<K, TC, GRAD>& L, int& ne, b
         found = found || mt;
     }
     const bool app = valid && !found;
+    if (!__any(app)) return;
     over = over || (app && ne >= K);
 #pragma unroll U
     for (int t = 0; t < K; ++t) {
@@ -346,6 +339,8 @@ __device__ __forceinline__ int merge_neighbours(const StepArgs& A, const Box& B,
     int ne = L.n0;
     bool over = false;
     const int ip_end = A.dnx ? 3 : 2;
+    // Neighbour values are read as stored: FP32 lists store every value, the implicit entry's as
+    // well (only the cell's own update recovers it as 1 - sum of the others in FP64).
 #pragma unroll 1
     for (int ip = A.dnx ? 0 : 1; ip < ip_end; ++ip) {
         // the plane's count bytes and row starts (selects: ip is a loop variable)
@@ -362,16 +357,18 @@ __device__ __forceinline__ int merge_neighbours(const StepArgs& A, const Box& B,
         for (int j = 0; j < 9; ++j) {
             const int m = ListCode<TV>::count(pc[j]), imp = ListCode<TV>::implicit(pc[j]);
             const uint32_t nb = prow[j / 3] + B.zs[j % 3];
+            // a lone implicit entry (FP32) has no stored value: it is 1
+            const bool lone = ListCode<TV>::kImplicit && m == 1 && imp == 0;
             g0[j] = buffer_load<uint16_t>(ri0, m > 0 ? nb * 2u : kNoLoad);
-            v0[j] = buffer_load<TV>(rv0, m > 0 && imp != 0 ? nb * SV : kNoLoad);
+            v0[j] = lone ? TV(1) : buffer_load<TV>(rv0, m > 0 ? nb * SV : kNoLoad);
             g1[j] = buffer_load<uint16_t>(ri1, m > 1 ? nb * 2u : kNoLoad);
-            v1[j] = buffer_load<TV>(rv1, m > 1 && imp != 1 ? nb * SV : kNoLoad);
+            v1[j] = buffer_load<TV>(rv1, m > 1 ? nb * SV : kNoLoad);
         }
         const int dx = ip - 1;
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
             const int jj = j / 3 - 1, kk = j % 3 - 1;
-            const int m = ListCode<TV>::count(pc[j]), imp = ListCode<TV>::implicit(pc[j]);
+            const int m = ListCode<TV>::count(pc[j]);
             const uint32_t nb = prow[j / 3] + B.zs[j % 3];
             const int nj = (jj != 0) + (kk != 0);
             const bool side = dx != 0 || nj != 0;
@@ -380,44 +377,26 @@ __device__ __forceinline__ int merge_neighbours(const StepArgs& A, const Box& B,
             const TC gwx = dx == 0 ? TC(0) : (dx < 0 ? -gw : gw);
             const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
             const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
-            // third entry (rare) before anything else: the implicit value may need it
+            // entries beyond the second (rare), loaded on demand
             TV v2 = TV(0);
             uint32_t g2 = 0u;
             if (m > 2) {
                 g2 = A.id_in[2 * N + nb];
-                if (imp != 2) v2 = load_val<TV>(A.val_in, 2 * N + nb);
+                v2 = load_val<TV>(A.val_in, 2 * N + nb);
             }
-            // implicit entry: 1 - sum of the others in slot order
-            double vimp = 0.0;
-            if (ListCode<TV>::kImplicit) {
-                if (m > 3) {
-                    vimp = implicit_value<TV>(A, nb, m, imp);
-                } else {
-                    double sum = 0.0;
-                    if (m > 0 && imp != 0) sum += static_cast<double>(v0[j]);
-                    if (m > 1 && imp != 1) sum += static_cast<double>(v1[j]);
-                    if (m > 2 && imp != 2) sum += static_cast<double>(v2);
-                    vimp = 1.0 - sum;
-                }
-            }
-            auto value = [&](int q, TV raw) -> TC {
-                return (ListCode<TV>::kImplicit && q == imp) ? static_cast<TC>(vimp) : static_cast<TC>(raw);
-            };
-            const TC a0 = value(0, v0[j]), a1 = value(1, v1[j]);
+            const TC a0 = static_cast<TC>(v0[j]), a1 = static_cast<TC>(v1[j]);
             merge_entry<K, TC, GRAD>(L, ne, over, m > 0 && a0 != TC(0), g0[j], w * a0,
                                      gwx * a0, gwy * a0, gwz * a0, side);
             merge_entry<K, TC, GRAD>(L, ne, over, m > 1 && a1 != TC(0), g1[j], w * a1,
                                      gwx * a1, gwy * a1, gwz * a1, side);
             if (m > 2) {
-                const TC a2 = value(2, v2);
+                const TC a2 = static_cast<TC>(v2);
                 merge_entry<K, TC, GRAD>(L, ne, over, a2 != TC(0), g2, w * a2, gwx * a2, gwy * a2, gwz * a2, side);
-            }
-            if (m > 3) {
                 if (!MANY) {
-                    over = true;
+                    over = over || m > 3;
                 } else {
                     for (int q = 3; q < m; ++q) {
-                        const TC aq = value(q, load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
+                        const TC aq = static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
                         const uint32_t gq = A.id_in[static_cast<int64_t>(q) * N + nb];
                         merge_entry<K, TC, GRAD>(L, ne, over, aq != TC(0), gq, w * aq, gwx * aq, gwy * aq, gwz * aq, side);
                     }
