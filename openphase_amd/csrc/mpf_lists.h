@@ -176,7 +176,14 @@ template <typename TV> struct ListCode {
 template <int K, typename TC, bool GRAD> struct CellList {
     uint32_t eid[K];
     double phi[K];
-    TC ev[K];
+    // value in the stencil arithmetic: a separate copy only when that is not FP64
+    static constexpr bool kCopy = sizeof(TC) != sizeof(double);
+    TC ev_[kCopy ? K : 1];
+    __device__ TC ev(int t) const
+    {
+        if constexpr (kCopy) return ev_[t];
+        else return phi[t];
+    }
     TC lap[K];
     TC gr[GRAD ? K : 1][3];
     bool keep[K];
@@ -253,7 +260,8 @@ __device__ __forceinline__ int load_own(const StepArgs& A, int64_t cc, uint8_t c
     L.ne = n0;
 #pragma unroll U
     for (int t = 0; t < K; ++t) {
-        L.eid[t] = 0u; L.phi[t] = 0.0; L.ev[t] = TC(0); L.lap[t] = TC(0); L.keep[t] = false;
+        L.eid[t] = 0u; L.phi[t] = 0.0; L.lap[t] = TC(0); L.keep[t] = false;
+        if constexpr (CellList<K, TC, GRAD>::kCopy) L.ev_[t] = TC(0);
         if (GRAD) { L.gr[GRAD ? t : 0][0] = L.gr[GRAD ? t : 0][1] = L.gr[GRAD ? t : 0][2] = TC(0); }
     }
     if (n0 > K) return kCellDeferred;
@@ -275,7 +283,8 @@ __device__ __forceinline__ int load_own(const StepArgs& A, int64_t cc, uint8_t c
             if (t == imp0) L.phi[t] = 1.0 - others;
     }
 #pragma unroll U
-    for (int t = 0; t < K; ++t) L.ev[t] = static_cast<TC>(L.phi[t]);
+    for (int t = 0; t < K; ++t)
+        if constexpr (CellList<K, TC, GRAD>::kCopy) L.ev_[t] = static_cast<TC>(L.phi[t]);
     return kCellWide;
 }
 
