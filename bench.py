@@ -97,6 +97,26 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
     }
 
 
+def profiled_traffic(kernel: str = "step_kernel<double, double, false, false, true>"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary (FETCH_SIZE and
+    WRITE_SIZE in KiB per dispatch, scaled by the factors the copy kernel gives: 2.00 and 1.00).
+    None when the summary does not list the kernel."""
+    path = ROOT / "profiles" / "mpf_step_256cube_1000grains.md"
+    if not path.exists():
+        return None
+    fetch = write = None
+    for line in path.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 3 and cols[0] == f"void {kernel}":
+            if cols[1] == "FETCH_SIZE":
+                fetch = float(cols[2])
+            elif cols[1] == "WRITE_SIZE":
+                write = float(cols[2])
+    if fetch is None or write is None:
+        return None
+    return fetch * 1024 * 2.0 + write * 1024 * 1.0
+
+
 def run_adjoint(steps: int, segment: int):
     """Differentiable rollout of the calibration workload: 2D 512^2 periodic, 200 grains of two
     phases (random half/half), uniform random cubic orientations, cubic anisotropy 0.05,
@@ -238,7 +258,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (ach / HBM_PEAK_GBS) if ach else None,
-            "traffic": None,
+            "traffic": profiled_traffic(),
         },
         "fp32": {
             "cell_updates_per_s": r32["cell_updates_per_s"],
