@@ -190,6 +190,25 @@ mpf_status mpf_restore_state(mpf_ctx* ctx, const void* device_src);
  * entries (MPF_ERR_UNSUPPORTED otherwise). */
 mpf_status mpf_adjoint_step(mpf_ctx* ctx, double dt, const double* lam_next, double* lam_prev, double* grad);
 
+/* Launch-shape autotuner (no reference counterpart): times `steps` steps of each candidate launch
+ * shape - XCD-contiguous block order of the step kernel on/off, 256 / 1024 / 4096 blocks for the
+ * deferred-cell kernels - on the context's own state (saved and restored around every trial, so
+ * the state is unchanged afterwards) and keeps the fastest. Results do not depend on the shape.
+ * The chosen shape is returned through the optional out pointers. */
+mpf_status mpf_autotune(mpf_ctx* ctx, double dt, int32_t steps, int32_t* xcd_order, int32_t* deferred_grid);
+mpf_status mpf_set_launch_shape(mpf_ctx* ctx, int32_t xcd_order, int32_t deferred_grid);
+
+/* Ensembles of independent domains (no reference counterpart; calibration runs over many uneven
+ * microstructures): the members (initialized contexts, each with its own grid, tables and dt)
+ * advance together. Two steps of every member are captured into one HIP graph (one branch per
+ * member) and replayed nsteps/2 times; an odd last step runs directly. A changed dt or member
+ * parity recaptures the graph. mpf_ensemble_step returns after all members finished and reports
+ * the first member error. Members must not be stepped elsewhere while an ensemble step runs. */
+typedef struct mpf_ensemble mpf_ensemble;
+mpf_status mpf_ensemble_create(mpf_ctx* const* members, int32_t n, mpf_ensemble** out);
+mpf_status mpf_ensemble_step(mpf_ensemble* ens, const double* dt, int32_t nsteps);
+void mpf_ensemble_destroy(mpf_ensemble* ens);
+
 /* Human-readable description of the last error on this context (static storage per context). */
 const char* mpf_error_message(mpf_ctx* ctx);
 

@@ -27,8 +27,9 @@ EXPORTED = [
     "mpf_step_async", "mpf_synchronize", "mpf_total_entries", "mpf_download", "mpf_get_grains",
     "mpf_max_time_step", "mpf_limiter_warnings", "mpf_overflow_cell", "mpf_device_state",
     "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
-    "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_error_message",
-    "mpf_destroy",
+    "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_autotune",
+    "mpf_set_launch_shape", "mpf_ensemble_create", "mpf_ensemble_step", "mpf_ensemble_destroy",
+    "mpf_error_message", "mpf_destroy",
 ]
 
 
@@ -98,6 +99,11 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_save_state": ([vp, vp], C.c_int),
         "mpf_restore_state": ([vp, vp], C.c_int),
         "mpf_adjoint_step": ([vp, C.c_double, vp, vp, vp], C.c_int),
+        "mpf_autotune": ([vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+        "mpf_set_launch_shape": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "mpf_ensemble_create": ([C.POINTER(vp), C.c_int32, C.POINTER(vp)], C.c_int),
+        "mpf_ensemble_step": ([vp, f64p, C.c_int32], C.c_int),
+        "mpf_ensemble_destroy": ([vp], None),
         "mpf_error_message": ([vp], C.c_char_p),
         "mpf_destroy": ([vp], None),
     }

@@ -100,6 +100,9 @@ struct mpf_ctx {
     unsigned* d_work_count = nullptr;
     unsigned long long* d_stats = nullptr;
 
+    int xcd_order = 1;               // launch shape, see StepArgs
+    int deferred_grid = mpf::kWideGrid;
+
     bool have_state = false, have_grains = false, initialized = false;
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -118,6 +121,17 @@ struct mpf_ctx {
         return MPF_ERR_HIP;
     }
     size_t vsize() const { return fp32 ? sizeof(float) : sizeof(double); }
+};
+
+struct mpf_ensemble {
+    std::vector<mpf_ctx*> members;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<double> dt;          // per member, as captured
+    std::vector<int> cur;            // per member buffer parity, as captured
+    std::vector<hipEvent_t> fork, join;
+    std::string err;
 };
 
 namespace {
@@ -159,6 +173,8 @@ mpf::StepArgs step_args(const mpf_ctx* c, double dt, int src)
     a.warnings = c->d_warnings;
     a.work_list = c->d_work_list;
     a.work_count = c->d_work_count;
+    a.xcd_order = c->xcd_order;
+    a.deferred_grid = c->deferred_grid;
     return a;
 }
 
@@ -176,6 +192,20 @@ mpf::GrainArgs grain_args(const mpf_ctx* c)
     g.work_count = c->d_work_count;
     g.stats = c->d_stats;
     return g;
+}
+
+// Enqueues one step (step kernel, deferred-cell kernels, grain kernel) on `stream` and flips the
+// double buffer. Kernels only: safe to capture in a graph.
+mpf_status enqueue_step(mpf_ctx* c, double dt, hipStream_t stream)
+{
+    const mpf::StepArgs a = step_args(c, dt, c->cur);
+    const mpf::GrainArgs g = grain_args(c);
+    const bool full = c->grad || c->dg;
+    mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_step(a, c->fp32, full, stream)), "step kernel");
+    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(a, c->fp32, full, stream)), "deferred-cell kernels");
+    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_grains(g, stream)), "grain kernel");
+    if (!st) c->cur = 1 - c->cur;
+    return st;
 }
 
 mpf_status read_status(mpf_ctx* c)
@@ -857,6 +887,174 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
     if (!st) st = c->hip(hipStreamSynchronize(c->stream), "stream synchronize");
     if (!st && words[3]) st = c->fail(MPF_ERR_UNSUPPORTED, "a cell's merged list exceeds the adjoint's 8 entries");
     return st;
+}
+
+// ------------------------------------------------------------------ launch-shape autotuner
+
+mpf_status mpf_autotune(mpf_ctx* c, double dt, int32_t steps, int32_t* xcd_order, int32_t* deferred_grid)
+{
+    if (!c || steps < 1 || !(dt > 0.0)) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
+    // time each candidate on the context's own state, restoring it before every trial
+    int64_t nbytes = 0;
+    if (mpf_status st = mpf_state_bytes(c, &nbytes)) return st;
+    void* saved = nullptr;
+    if (c->hip(hipMalloc(&saved, static_cast<size_t>(nbytes)), "autotune checkpoint")) return MPF_ERR_NOMEM;
+    const int cur0 = c->cur;
+    mpf_status st = mpf_save_state(c, saved);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st) st = c->hip(hipEventCreate(&e0), "hipEventCreate");
+    if (!st) st = c->hip(hipEventCreate(&e1), "hipEventCreate");
+    const int grids[3] = {256, 1024, mpf::kWideGrid};
+    float best = 1e30f;
+    int best_x = c->xcd_order, best_g = c->deferred_grid;
+    for (int x = 0; x < 2 && !st; ++x)
+        for (int gi = 0; gi < 3 && !st; ++gi) {
+            c->xcd_order = x;
+            c->deferred_grid = grids[gi];
+            c->cur = cur0;
+            st = mpf_restore_state(c, saved);
+            if (!st) st = enqueue_step(c, dt, c->stream);  // warm-up
+            if (!st) st = c->hip(hipEventRecord(e0, c->stream), "hipEventRecord");
+            for (int k = 0; k < steps && !st; ++k) st = enqueue_step(c, dt, c->stream);
+            if (!st) st = c->hip(hipEventRecord(e1, c->stream), "hipEventRecord");
+            if (!st) st = c->hip(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            if (!st) st = c->hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (!st) st = read_status(c);
+            if (!st && ms < best) { best = ms; best_x = x; best_g = grids[gi]; }
+        }
+    c->xcd_order = best_x;
+    c->deferred_grid = best_g;
+    c->cur = cur0;
+    if (!st) st = mpf_restore_state(c, saved);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(saved);
+    if (xcd_order) *xcd_order = best_x;
+    if (deferred_grid) *deferred_grid = best_g;
+    return st;
+}
+
+mpf_status mpf_set_launch_shape(mpf_ctx* c, int32_t xcd_order, int32_t deferred_grid)
+{
+    if (!c || deferred_grid < 1) return MPF_ERR_ARG;
+    c->xcd_order = xcd_order ? 1 : 0;
+    c->deferred_grid = deferred_grid;
+    return MPF_OK;
+}
+
+// ------------------------------------------------------------------ ensembles
+
+namespace {
+
+void drop_graph(mpf_ensemble* e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+// Captures two steps of every member (each on its own stream, forked from and joined back into
+// the ensemble stream) into one graph; after the two steps every member is back on its buffer.
+mpf_status capture(mpf_ensemble* e, const double* dt)
+{
+    drop_graph(e);
+    const size_t n = e->members.size();
+    hipError_t h = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (h != hipSuccess) { e->err = "hipStreamBeginCapture failed"; return MPF_ERR_HIP; }
+    mpf_status st = MPF_OK;
+    for (size_t i = 0; i < n && !st; ++i) {
+        mpf_ctx* c = e->members[i];
+        if (hipEventRecord(e->fork[i], e->stream) != hipSuccess || hipStreamWaitEvent(c->stream, e->fork[i], 0) != hipSuccess) {
+            st = MPF_ERR_HIP;
+            break;
+        }
+        st = enqueue_step(c, dt[i], c->stream);
+        if (!st) st = enqueue_step(c, dt[i], c->stream);
+        if (!st && (hipEventRecord(e->join[i], c->stream) != hipSuccess || hipStreamWaitEvent(e->stream, e->join[i], 0) != hipSuccess))
+            st = MPF_ERR_HIP;
+    }
+    hipGraph_t g = nullptr;
+    h = hipStreamEndCapture(e->stream, &g);
+    if (st || h != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        e->err = "ensemble capture failed";
+        return st ? st : MPF_ERR_HIP;
+    }
+    e->graph = g;
+    if (hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        drop_graph(e);
+        e->err = "hipGraphInstantiate failed";
+        return MPF_ERR_HIP;
+    }
+    e->dt.assign(dt, dt + n);
+    e->cur.resize(n);
+    for (size_t i = 0; i < n; ++i) e->cur[i] = e->members[i]->cur;
+    return MPF_OK;
+}
+
+}  // namespace
+
+mpf_status mpf_ensemble_create(mpf_ctx* const* members, int32_t n, mpf_ensemble** out)
+{
+    if (!members || n < 1 || !out) return MPF_ERR_ARG;
+    *out = nullptr;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!members[i] || !members[i]->initialized) return MPF_ERR_STATE;
+        for (int32_t j = 0; j < i; ++j)
+            if (members[j] == members[i]) return MPF_ERR_ARG;
+    }
+    mpf_ensemble* e = new (std::nothrow) mpf_ensemble();
+    if (!e) return MPF_ERR_NOMEM;
+    e->members.assign(members, members + n);
+    bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
+    e->fork.assign(n, nullptr);
+    e->join.assign(n, nullptr);
+    for (int32_t i = 0; i < n && ok; ++i)
+        ok = hipEventCreateWithFlags(&e->fork[i], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        mpf_ensemble_destroy(e);
+        return MPF_ERR_HIP;
+    }
+    *out = e;
+    return MPF_OK;
+}
+
+mpf_status mpf_ensemble_step(mpf_ensemble* e, const double* dt, int32_t nsteps)
+{
+    if (!e || !dt || nsteps < 0) return MPF_ERR_ARG;
+    const size_t n = e->members.size();
+    for (size_t i = 0; i < n; ++i)
+        if (!(dt[i] > 0.0)) return MPF_ERR_ARG;
+    // members' earlier work on their own streams precedes the graph
+    for (size_t i = 0; i < n; ++i)
+        if (hipEventRecord(e->join[i], e->members[i]->stream) != hipSuccess || hipStreamWaitEvent(e->stream, e->join[i], 0) != hipSuccess)
+            return MPF_ERR_HIP;
+    bool stale = e->exec == nullptr;
+    for (size_t i = 0; i < n && !stale; ++i)
+        stale = e->dt[i] != dt[i] || e->cur[i] != e->members[i]->cur;
+    mpf_status st = MPF_OK;
+    if (nsteps >= 2 && stale) st = capture(e, dt);
+    for (int32_t r = 0; r < nsteps / 2 && !st; ++r)
+        if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) { e->err = "hipGraphLaunch failed"; st = MPF_ERR_HIP; }
+    if (!st && hipStreamSynchronize(e->stream) != hipSuccess) st = MPF_ERR_HIP;
+    if (!st && (nsteps & 1))
+        for (size_t i = 0; i < n && !st; ++i) st = enqueue_step(e->members[i], dt[i], e->members[i]->stream);
+    for (size_t i = 0; i < n && !st; ++i) st = read_status(e->members[i]);
+    return st;
+}
+
+void mpf_ensemble_destroy(mpf_ensemble* e)
+{
+    if (!e) return;
+    drop_graph(e);
+    for (auto ev : e->fork) if (ev) (void)hipEventDestroy(ev);
+    for (auto ev : e->join) if (ev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
 }
 
 }  // extern "C"

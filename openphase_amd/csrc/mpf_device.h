@@ -59,6 +59,10 @@ struct StepArgs {
     unsigned long long* warnings;
     uint32_t* work_list;         // deferred cells: four lists of ncells each (mid kernel levels 0-2; wide kernel)
     unsigned* work_count;        // [4] lengths of the four lists
+    // launch shape (chosen by mpf_autotune): XCD-contiguous block order of the step kernel, and
+    // the number of blocks of the grid-stride deferred-cell kernels
+    int32_t xcd_order;
+    int32_t deferred_grid;
 };
 
 struct GrainArgs {

@@ -490,7 +490,8 @@ __global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A)
     __shared__ uint16_t s_list[kBlock];
     __shared__ uint32_t s_wide[kWaves];
     const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
-    const int64_t base = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBlock;
+    const unsigned blk = A.xcd_order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int64_t base = static_cast<int64_t>(blk) * kBlock;
     const int64_t c = base + tid;
     const bool valid = c < A.ncells;
     bool bad_volume = false;
@@ -859,7 +860,7 @@ int launch_step(const StepArgs& a, bool fp32, bool full, void* stream)
 
 int launch_deferred(const StepArgs& a, bool fp32, bool full, void* stream)
 {
-    const dim3 grid(kWideGrid);
+    const dim3 grid(static_cast<unsigned>(a.deferred_grid > 0 ? a.deferred_grid : kWideGrid));
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (fp32) {
         if (full) {

@@ -242,3 +242,47 @@ class Engine:
         """One adjoint step from the current state (device pointers, see mpf_adjoint_step)."""
         self._check(self.lib.mpf_adjoint_step(self.h, float(dt), C.c_void_p(lam_next), C.c_void_p(lam_prev),
                                               C.c_void_p(grad)), "mpf_adjoint_step")
+
+    # ---------------------------------------------------------------- launch shape
+    def autotune(self, dt: float, steps: int = 3):
+        """Times the candidate launch shapes on this state (restored afterwards) and keeps the
+        fastest; returns (xcd_order, deferred_grid)."""
+        x, g = C.c_int32(), C.c_int32()
+        self._check(self.lib.mpf_autotune(self.h, float(dt), int(steps), C.byref(x), C.byref(g)), "mpf_autotune")
+        return x.value, g.value
+
+    def set_launch_shape(self, xcd_order: int, deferred_grid: int) -> None:
+        self._check(self.lib.mpf_set_launch_shape(self.h, int(xcd_order), int(deferred_grid)), "mpf_set_launch_shape")
+
+
+class Ensemble:
+    """Independent domains advanced together: two steps of every member per captured HIP graph
+    (include/mpf.h, mpf_ensemble_*)."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        self.lib = _capi.load()
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        st = self.lib.mpf_ensemble_create(arr, len(self.engines), C.byref(h))
+        if st != _capi.MPF_OK:
+            raise MpfError(st, "mpf_ensemble_create", "")
+        self.h = h
+
+    def step(self, dt, nsteps: int) -> None:
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (len(self.engines),)))
+        st = self.lib.mpf_ensemble_step(self.h, ptr(d, C.c_double), int(nsteps))
+        if st != _capi.MPF_OK:
+            msgs = [e.lib.mpf_error_message(e.h) for e in self.engines]
+            raise MpfError(st, "mpf_ensemble_step", "; ".join(m.decode() for m in msgs if m))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mpf_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
