@@ -178,6 +178,60 @@ def run_adjoint(steps: int, segment: int):
     }
 
 
+def run_ensemble(steps: int):
+    """Calibration ensemble: 64 periodic 2D domains, side round(128 * 4^U) with U ~ U[0,1] (128 to
+    512), 20-300 grains (log-uniform), sigma ~ U[0.1, 0.5] J/m^2 and mu ~ logU[1e-9, 1e-8] per
+    domain, FP32, dt = 0.9 x each domain's limit; seed 0x0FE6. Each member is autotuned, then the
+    ensemble (one captured graph per two steps of all members) and the same domains stepped one
+    after another are timed. Returns total cell-updates/s of both."""
+    from openphase_amd.engine import Engine, Ensemble, StepConfig
+    from openphase_amd.microstructure import voronoi_grains
+
+    rng = np.random.default_rng(0x0FE6)
+    engines, dts, cells = [], [], 0
+    for i in range(64):
+        side = int(round(128 * 4 ** rng.uniform()))
+        ng = int(round(np.exp(rng.uniform(np.log(20), np.log(300)))))
+        sig = rng.uniform(0.1, 0.5)
+        mu = float(np.exp(rng.uniform(np.log(1e-9), np.log(1e-8))))
+        cfg = StepConfig(side, side, 0, sigma=np.array([[sig]]), mu=np.array([[mu]]), precision=1, slots=8)
+        e = Engine(cfg)
+        e.set_grains(np.zeros(ng, np.int32))
+        e.upload(voronoi_grains(side, side, 0, ng, seed=0x0FE6 + i))
+        e.finalize_initialization()
+        dt = 0.9 * cfg.max_time_step()
+        e.step(dt, 2)
+        e.autotune(dt, steps=2)
+        engines.append(e)
+        dts.append(dt)
+        cells += side * side
+    ens = Ensemble(engines)
+    ens.step(dts, 2)
+    _device_sync()
+    t0 = time.perf_counter()
+    ens.step(dts, steps)
+    _device_sync()
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for e, dt in zip(engines, dts):
+            e.step(dt, 1)
+    _device_sync()
+    t_seq = time.perf_counter() - t0
+    ens.close()
+    for e in engines:
+        e.close()
+    return {
+        "cell_updates_per_s": cells * steps / t_graph,
+        "sequential_cell_updates_per_s": cells * steps / t_seq,
+        "domains": 64,
+        "cells": cells,
+        "steps": steps,
+        "workload": "64 periodic 2D domains, sides 128-512, 20-300 grains, per-domain sigma/mu/dt, FP32; "
+                    "one captured graph per two steps of all members vs the domains stepped one by one",
+    }
+
+
 def cpu_baseline(lists, budget_s: float = 30.0):
     """Times the reference OpenPhase build (oracle/_ref/ref_driver, compiled from the reference
     sources) on this host: same 256^3 state, one step, all available cores (max 16)."""
@@ -212,6 +266,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--adjoint-steps", type=int, default=100, help="rollout length of the adjoint leg")
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the ensemble leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the step on a single MI355X "
@@ -229,6 +284,8 @@ def main():
     print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
     adj = run_adjoint(args.adjoint_steps, 25)
     print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
+    ens = run_ensemble(args.ensemble_steps)
+    print(f"[bench] ensemble {ens}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(lists)
     print(f"[bench] cpu {cpu}", file=sys.stderr, flush=True)
 
@@ -277,6 +334,7 @@ def main():
             "copy_kernel_gbs": r64["copy_gbs"],
         },
         "adjoint": adj,
+        "ensemble": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
