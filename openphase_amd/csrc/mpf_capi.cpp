@@ -103,6 +103,15 @@ struct mpf_ctx {
     int xcd_order = 1;               // launch shape, see StepArgs
     int deferred_grid = mpf::kWideGrid;
 
+    // device copies of the kernel arguments (both buffer parities), uploaded when they change
+    mpf::StepArgs h_args[2]{};
+    mpf::GrainArgs h_gargs{};
+    bool args_ok = false;
+    mpf::StepArgs* d_args = nullptr;
+    uint32_t* d_first = nullptr;
+    mpf::GrainArgs* d_gargs = nullptr;
+    uint32_t* d_gfirst = nullptr;
+
     bool have_state = false, have_grains = false, initialized = false;
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -123,14 +132,29 @@ struct mpf_ctx {
     size_t vsize() const { return fp32 ? sizeof(float) : sizeof(double); }
 };
 
+// Members of one kernel variant (precision, orientation/driving-force terms, equal-energy form)
+// share every launch: device arrays of their arguments for the two captured steps.
+struct mpf_group {
+    bool fp32 = false, full = false, uni = false;
+    std::vector<int> members;
+    std::vector<mpf::StepArgs> h_a1, h_a2;
+    std::vector<mpf::GrainArgs> h_g;
+    std::vector<uint32_t> h_first, h_gfirst;
+    mpf::StepArgs* d_a1 = nullptr;
+    mpf::StepArgs* d_a2 = nullptr;
+    mpf::GrainArgs* d_g = nullptr;
+    uint32_t* d_first = nullptr;
+    uint32_t* d_gfirst = nullptr;
+    int per_domain = 256;
+};
+
 struct mpf_ensemble {
     std::vector<mpf_ctx*> members;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<double> dt;          // per member, as captured
-    std::vector<int> cur;            // per member buffer parity, as captured
-    std::vector<hipEvent_t> fork, join;
+    std::vector<mpf_group> groups;
+    std::vector<hipEvent_t> join;
     std::string err;
 };
 
@@ -194,16 +218,63 @@ mpf::GrainArgs grain_args(const mpf_ctx* c)
     return g;
 }
 
+unsigned step_blocks(const mpf_ctx* c) { return static_cast<unsigned>((c->ncells + mpf::kBlock - 1) / mpf::kBlock); }
+unsigned grain_blocks(const mpf_ctx* c)
+{
+    const int n = c->ngrains > 0 ? c->ngrains : 1;
+    return static_cast<unsigned>((n + mpf::kGrainBlock - 1) / mpf::kGrainBlock);
+}
+
+// Makes the device copies of the kernel arguments current for time step dt (both parities).
+// Uploads only when something changed (dt, tables, launch shape, grains); then it waits for the
+// copies, so the host arrays can be reused.
+mpf_status sync_args(mpf_ctx* c, double dt, hipStream_t stream)
+{
+    const mpf::StepArgs a[2] = {step_args(c, dt, 0), step_args(c, dt, 1)};
+    const mpf::GrainArgs g = grain_args(c);
+    if (c->args_ok && std::memcmp(a, c->h_args, sizeof(a)) == 0 && std::memcmp(&g, &c->h_gargs, sizeof(g)) == 0)
+        return MPF_OK;
+    if (!c->d_args) {
+        if (c->hip(dalloc(&c->d_args, 2), "argument blocks") || c->hip(dalloc(&c->d_first, 2), "argument blocks") ||
+            c->hip(dalloc(&c->d_gargs, 1), "argument blocks") || c->hip(dalloc(&c->d_gfirst, 2), "argument blocks"))
+            return MPF_ERR_NOMEM;
+    }
+    std::memcpy(c->h_args, a, sizeof(a));
+    c->h_gargs = g;
+    const uint32_t first[2] = {0u, step_blocks(c)}, gfirst[2] = {0u, grain_blocks(c)};
+    hipError_t e = hipMemcpyAsync(c->d_args, c->h_args, sizeof(a), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_first, first, sizeof(first), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_gargs, &c->h_gargs, sizeof(g), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_gfirst, gfirst, sizeof(gfirst), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    c->args_ok = e == hipSuccess;
+    return c->hip(e, "argument upload");
+}
+
+mpf_status launch_step_part(mpf_ctx* c, hipStream_t stream)
+{
+    const bool full = c->grad || c->dg;
+    return c->hip(static_cast<hipError_t>(mpf::launch_step(c->h_args[c->cur], nullptr, nullptr, 1, step_blocks(c), c->fp32,
+                                                           full, c->uniform, stream)), "step kernel");
+}
+
+mpf_status launch_rest_part(mpf_ctx* c, hipStream_t stream)
+{
+    const bool full = c->grad || c->dg;
+    mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(c->d_args + c->cur, 1, c->deferred_grid, c->fp32,
+                                                                        full, stream)), "deferred-cell kernels");
+    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_grains(c->d_gargs, c->d_gfirst, 1, grain_blocks(c), stream)),
+                         "grain kernel");
+    return st;
+}
+
 // Enqueues one step (step kernel, deferred-cell kernels, grain kernel) on `stream` and flips the
-// double buffer. Kernels only: safe to capture in a graph.
+// double buffer.
 mpf_status enqueue_step(mpf_ctx* c, double dt, hipStream_t stream)
 {
-    const mpf::StepArgs a = step_args(c, dt, c->cur);
-    const mpf::GrainArgs g = grain_args(c);
-    const bool full = c->grad || c->dg;
-    mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_step(a, c->fp32, full, stream)), "step kernel");
-    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(a, c->fp32, full, stream)), "deferred-cell kernels");
-    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_grains(g, stream)), "grain kernel");
+    mpf_status st = sync_args(c, dt, stream);
+    if (!st) st = launch_step_part(c, stream);
+    if (!st) st = launch_rest_part(c, stream);
     if (!st) c->cur = 1 - c->cur;
     return st;
 }
@@ -469,7 +540,10 @@ mpf_status mpf_finalize_initialization(mpf_ctx* c)
     mpf::StepArgs a = step_args(c, 0.0, c->cur);
     mpf_status s = c->hip(static_cast<hipError_t>(mpf::launch_finalize_init(a, c->fp32, c->stream)), "finalize kernel");
     if (s) return s;
-    s = c->hip(static_cast<hipError_t>(mpf::launch_grains(grain_args(c), c->stream)), "grain kernel");
+    s = sync_args(c, 1.0, c->stream);
+    if (s) return s;
+    s = c->hip(static_cast<hipError_t>(mpf::launch_grains(c->d_gargs, c->d_gfirst, 1, grain_blocks(c), c->stream)),
+               "grain kernel");
     if (s) return s;
     c->cur = 1 - c->cur;
     s = read_status(c);
@@ -495,14 +569,13 @@ mpf_status mpf_step_async(mpf_ctx* c, double dt, int32_t nsteps)
     auto record = [&](int i) -> mpf_status {
         return timing ? c->hip(hipEventRecord(c->events[i], c->stream), "hipEventRecord") : MPF_OK;
     };
-    const mpf::GrainArgs g = grain_args(c);
+    if (nsteps > 0)
+        if (mpf_status st = sync_args(c, dt, c->stream)) return st;
     for (int s = 0; s < nsteps; ++s) {
-        mpf::StepArgs a = step_args(c, dt, c->cur);
         mpf_status st = record(3 * s);
-        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_step(a, c->fp32, c->grad || c->dg, c->stream)), "step kernel");
+        if (!st) st = launch_step_part(c, c->stream);
         if (!st) st = record(3 * s + 1);
-        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(a, c->fp32, c->grad || c->dg, c->stream)), "deferred-cell kernels");
-        if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_grains(g, c->stream)), "grain kernel");
+        if (!st) st = launch_rest_part(c, c->stream);
         if (!st) st = record(3 * s + 2);
         if (st) return st;
         c->cur = 1 - c->cur;
@@ -730,6 +803,7 @@ void mpf_destroy(mpf_ctx* c)
     for (int b = 0; b < 2; ++b) { dfree(c->cnt[b]); dfree(c->id[b]); dfree(c->val[b]); }
     free_grains(c);
     dfree(c->a_lam_own); dfree(c->a_cnt); dfree(c->a_id); dfree(c->a_lap); dfree(c->a_grad);
+    dfree(c->d_args); dfree(c->d_first); dfree(c->d_gargs); dfree(c->d_gfirst);
     dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
@@ -956,42 +1030,98 @@ void drop_graph(mpf_ensemble* e)
     e->graph = nullptr;
 }
 
-// Captures two steps of every member (each on its own stream, forked from and joined back into
-// the ensemble stream) into one graph; after the two steps every member is back on its buffer.
-mpf_status capture(mpf_ensemble* e, const double* dt)
+void free_group(mpf_group& g)
+{
+    dfree(g.d_a1); dfree(g.d_a2); dfree(g.d_g); dfree(g.d_first); dfree(g.d_gfirst);
+}
+
+// Host argument arrays of every group for the next two steps (members at their current parity).
+// Returns true when they differ from what the device arrays hold.
+bool build_args(mpf_ensemble* e, const double* dt)
+{
+    bool changed = false;
+    for (auto& g : e->groups) {
+        const size_t n = g.members.size();
+        std::vector<mpf::StepArgs> a1(n), a2(n);
+        std::vector<mpf::GrainArgs> ga(n);
+        std::vector<uint32_t> first(n + 1, 0u), gfirst(n + 1, 0u);
+        int per = 1;
+        for (size_t k = 0; k < n; ++k) {
+            const mpf_ctx* c = e->members[g.members[k]];
+            const double d = dt[g.members[k]];
+            a1[k] = step_args(c, d, c->cur);
+            a2[k] = step_args(c, d, 1 - c->cur);
+            a1[k].xcd_order = a2[k].xcd_order = 1;
+            ga[k] = grain_args(c);
+            first[k + 1] = first[k] + step_blocks(c);
+            gfirst[k + 1] = gfirst[k] + grain_blocks(c);
+            per = std::max(per, std::min(c->deferred_grid, 1024));
+        }
+        auto differs = [](const void* x, const void* y, size_t bytes) { return std::memcmp(x, y, bytes) != 0; };
+        if (g.h_a1.size() != n || differs(a1.data(), g.h_a1.data(), n * sizeof(mpf::StepArgs)) ||
+            differs(a2.data(), g.h_a2.data(), n * sizeof(mpf::StepArgs)) ||
+            differs(ga.data(), g.h_g.data(), n * sizeof(mpf::GrainArgs)) || g.per_domain != per) {
+            changed = true;
+            g.h_a1 = std::move(a1);
+            g.h_a2 = std::move(a2);
+            g.h_g = std::move(ga);
+            g.h_first = std::move(first);
+            g.h_gfirst = std::move(gfirst);
+            g.per_domain = per;
+        }
+    }
+    return changed;
+}
+
+// Uploads the groups' argument arrays and captures two steps of every group into one graph:
+// per step and group one launch of each kernel, over all its members.
+mpf_status capture(mpf_ensemble* e)
 {
     drop_graph(e);
-    const size_t n = e->members.size();
-    hipError_t h = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
-    if (h != hipSuccess) { e->err = "hipStreamBeginCapture failed"; return MPF_ERR_HIP; }
-    mpf_status st = MPF_OK;
-    for (size_t i = 0; i < n && !st; ++i) {
-        mpf_ctx* c = e->members[i];
-        if (hipEventRecord(e->fork[i], e->stream) != hipSuccess || hipStreamWaitEvent(c->stream, e->fork[i], 0) != hipSuccess) {
-            st = MPF_ERR_HIP;
-            break;
+    for (auto& g : e->groups) {
+        const size_t n = g.members.size();
+        free_group(g);
+        hipError_t h = dalloc(&g.d_a1, n);
+        if (h == hipSuccess) h = dalloc(&g.d_a2, n);
+        if (h == hipSuccess) h = dalloc(&g.d_g, n);
+        if (h == hipSuccess) h = dalloc(&g.d_first, n + 1);
+        if (h == hipSuccess) h = dalloc(&g.d_gfirst, n + 1);
+        if (h == hipSuccess) h = hipMemcpy(g.d_a1, g.h_a1.data(), n * sizeof(mpf::StepArgs), hipMemcpyHostToDevice);
+        if (h == hipSuccess) h = hipMemcpy(g.d_a2, g.h_a2.data(), n * sizeof(mpf::StepArgs), hipMemcpyHostToDevice);
+        if (h == hipSuccess) h = hipMemcpy(g.d_g, g.h_g.data(), n * sizeof(mpf::GrainArgs), hipMemcpyHostToDevice);
+        if (h == hipSuccess) h = hipMemcpy(g.d_first, g.h_first.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (h == hipSuccess) h = hipMemcpy(g.d_gfirst, g.h_gfirst.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (h != hipSuccess) { e->err = "ensemble argument upload failed"; return MPF_ERR_HIP; }
+    }
+    if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        e->err = "hipStreamBeginCapture failed";
+        return MPF_ERR_HIP;
+    }
+    int rc = 0;
+    for (int half = 0; half < 2 && rc == 0; ++half)
+        for (auto& g : e->groups) {
+            const int n = static_cast<int>(g.members.size());
+            mpf::StepArgs* args = half == 0 ? g.d_a1 : g.d_a2;
+            const unsigned blocks = g.h_first[n], gblocks = g.h_gfirst[n];
+            rc = mpf::launch_step(half == 0 ? g.h_a1[0] : g.h_a2[0], args, g.d_first, n, blocks, g.fp32, g.full, g.uni,
+                                  e->stream);
+            if (!rc) rc = mpf::launch_deferred(args, n, g.per_domain, g.fp32, g.full, e->stream);
+            if (!rc) rc = mpf::launch_grains(g.d_g, g.d_gfirst, n, gblocks, e->stream);
+            if (rc) break;
         }
-        st = enqueue_step(c, dt[i], c->stream);
-        if (!st) st = enqueue_step(c, dt[i], c->stream);
-        if (!st && (hipEventRecord(e->join[i], c->stream) != hipSuccess || hipStreamWaitEvent(e->stream, e->join[i], 0) != hipSuccess))
-            st = MPF_ERR_HIP;
-    }
-    hipGraph_t g = nullptr;
-    h = hipStreamEndCapture(e->stream, &g);
-    if (st || h != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
+    hipGraph_t graph = nullptr;
+    const hipError_t h = hipStreamEndCapture(e->stream, &graph);
+    if (rc || h != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
         e->err = "ensemble capture failed";
-        return st ? st : MPF_ERR_HIP;
+        return MPF_ERR_HIP;
     }
-    e->graph = g;
-    if (hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    e->graph = graph;
+    if (hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
         drop_graph(e);
         e->err = "hipGraphInstantiate failed";
         return MPF_ERR_HIP;
     }
-    e->dt.assign(dt, dt + n);
-    e->cur.resize(n);
-    for (size_t i = 0; i < n; ++i) e->cur[i] = e->members[i]->cur;
     return MPF_OK;
 }
 
@@ -1009,12 +1139,24 @@ mpf_status mpf_ensemble_create(mpf_ctx* const* members, int32_t n, mpf_ensemble*
     mpf_ensemble* e = new (std::nothrow) mpf_ensemble();
     if (!e) return MPF_ERR_NOMEM;
     e->members.assign(members, members + n);
+    for (int32_t i = 0; i < n; ++i) {
+        const mpf_ctx* c = members[i];
+        const bool full = c->grad || c->dg;
+        mpf_group* g = nullptr;
+        for (auto& q : e->groups)
+            if (q.fp32 == c->fp32 && q.full == full && q.uni == c->uniform) g = &q;
+        if (!g) {
+            e->groups.emplace_back();
+            g = &e->groups.back();
+            g->fp32 = c->fp32;
+            g->full = full;
+            g->uni = c->uniform;
+        }
+        g->members.push_back(i);
+    }
     bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
-    e->fork.assign(n, nullptr);
     e->join.assign(n, nullptr);
-    for (int32_t i = 0; i < n && ok; ++i)
-        ok = hipEventCreateWithFlags(&e->fork[i], hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
+    for (int32_t i = 0; i < n && ok; ++i) ok = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         mpf_ensemble_destroy(e);
         return MPF_ERR_HIP;
@@ -1033,11 +1175,8 @@ mpf_status mpf_ensemble_step(mpf_ensemble* e, const double* dt, int32_t nsteps)
     for (size_t i = 0; i < n; ++i)
         if (hipEventRecord(e->join[i], e->members[i]->stream) != hipSuccess || hipStreamWaitEvent(e->stream, e->join[i], 0) != hipSuccess)
             return MPF_ERR_HIP;
-    bool stale = e->exec == nullptr;
-    for (size_t i = 0; i < n && !stale; ++i)
-        stale = e->dt[i] != dt[i] || e->cur[i] != e->members[i]->cur;
     mpf_status st = MPF_OK;
-    if (nsteps >= 2 && stale) st = capture(e, dt);
+    if (nsteps >= 2 && (build_args(e, dt) || !e->exec)) st = capture(e);
     for (int32_t r = 0; r < nsteps / 2 && !st; ++r)
         if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) { e->err = "hipGraphLaunch failed"; st = MPF_ERR_HIP; }
     if (!st && hipStreamSynchronize(e->stream) != hipSuccess) st = MPF_ERR_HIP;
@@ -1050,8 +1189,9 @@ mpf_status mpf_ensemble_step(mpf_ensemble* e, const double* dt, int32_t nsteps)
 void mpf_ensemble_destroy(mpf_ensemble* e)
 {
     if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
-    for (auto ev : e->fork) if (ev) (void)hipEventDestroy(ev);
+    for (auto& g : e->groups) free_group(g);
     for (auto ev : e->join) if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;

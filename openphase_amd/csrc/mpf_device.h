@@ -98,12 +98,20 @@ constexpr int kAdjSlots = 8;     // merged entries per cell the adjoint handles
 
 int launch_adjoint(const AdjointArgs& a, void* stream);
 
-// Host-side launchers (mpf_kernels.hip); return a hipError_t value.
-// `full`: orientation-dependent energies and/or constant driving force present.
-int launch_step(const StepArgs& a, bool fp32, bool full, void* stream);
-int launch_deferred(const StepArgs& a, bool fp32, bool full, void* stream);
+constexpr int kGrainBlock = 256;                     // threads per block of the grain kernel
+
+// Host-side launchers (mpf_kernels.hip); return a hipError_t value. The step, deferred-cell and
+// grain kernels run over one or several domains of the same kernel variant in one launch: `args`
+// (device memory) holds one argument block per domain and `first` (device, ndom + 1 entries) the
+// prefix sums of the domains' block counts (the step kernel takes a single domain's arguments by
+// value in `one` when ndom is 1, and ignores `args` then); the deferred-cell kernels give every domain
+// `per_domain` blocks. `full`: orientation-dependent energies and/or constant driving force;
+// `uni`: the equal-energy form.
+int launch_step(const StepArgs& one, const StepArgs* args, const uint32_t* first, int ndom, unsigned blocks, bool fp32,
+                bool full, bool uni, void* stream);
+int launch_deferred(const StepArgs* args, int ndom, int per_domain, bool fp32, bool full, void* stream);
 int launch_finalize_init(const StepArgs& a, bool fp32, void* stream);
-int launch_grains(const GrainArgs& g, void* stream);
+int launch_grains(const GrainArgs* args, const uint32_t* first, int ndom, unsigned blocks, void* stream);
 int launch_copy(const void* src, void* dst, int64_t bytes, void* stream);
 
 }  // namespace mpf

@@ -407,6 +407,19 @@ __device__ __forceinline__ void report_overflow(const StepArgs& A, int64_t c, Ce
     atomicMin(A.first_overflow, static_cast<unsigned long long>(c));
 }
 
+// Domain of block b in a launch over several domains: first[d] <= b < first[d+1] (first holds
+// ndom + 1 prefix sums of the domains' block counts).
+__device__ __forceinline__ int batch_domain(const uint32_t* __restrict__ first, int ndom, unsigned b)
+{
+    int lo = 0, hi = ndom - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Workgroups are dispatched round-robin over the eight XCDs, each with its own L2. Renumbering
 // them so that every XCD walks one contiguous range of cells keeps the neighbour rows a block
 // reads mostly in the L2 that already holds them. A bijection of [0, nb).
@@ -481,17 +494,15 @@ __device__ __forceinline__ void wave_volume_add_lists(const bool keep[K], const 
 // wide cells are compacted in thread order into LDS. Phase 2 runs full wavefronts of wide cells
 // through merge, pair increments, limiter, merge and finalize, so bulk lanes no longer ride along
 // with interface lanes. Cells needing more than kMainSlots entries go to the mid kernel's list.
+constexpr int kStepWaves = kBlock / 64;
+
 template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
-__global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A)
+__device__ __forceinline__ void step_block(const StepArgs& A, int64_t base, uint32_t (&s_codes)[7][kBlock],
+                                           uint16_t (&s_list)[kBlock], uint32_t (&s_wide)[kStepWaves])
 {
     constexpr int K = kMainSlots;
-    constexpr int kWaves = kBlock / 64;
-    __shared__ uint32_t s_codes[7][kBlock];
-    __shared__ uint16_t s_list[kBlock];
-    __shared__ uint32_t s_wide[kWaves];
+    constexpr int kWaves = kStepWaves;
     const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
-    const unsigned blk = A.xcd_order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int64_t base = static_cast<int64_t>(blk) * kBlock;
     const int64_t c = base + tid;
     const bool valid = c < A.ncells;
     bool bad_volume = false;
@@ -601,6 +612,27 @@ __global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A)
     if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
 }
 
+// One domain (ndom <= 1): arguments by value in the kernel-argument segment, which the hardware
+// preloads into scalar registers. Several domains in one launch (ensembles): the argument table
+// and block prefix sums in device memory; blocks are XCD-ordered over the whole launch first.
+template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
+__global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A0, const StepArgs* __restrict__ batch,
+                                                          const uint32_t* __restrict__ first, int32_t ndom)
+{
+    __shared__ uint32_t s_codes[7][kBlock];
+    __shared__ uint16_t s_list[kBlock];
+    __shared__ uint32_t s_wide[kStepWaves];
+    if (ndom <= 1) {
+        const unsigned gb = A0.xcd_order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+        step_block<TV, TC, GRAD, DG, UNI>(A0, static_cast<int64_t>(gb) * kBlock, s_codes, s_list, s_wide);
+    } else {
+        const unsigned gb = batch[0].xcd_order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+        const int dom = batch_domain(first, ndom, gb);
+        step_block<TV, TC, GRAD, DG, UNI>(batch[dom], static_cast<int64_t>(gb - first[dom]) * kBlock, s_codes, s_list,
+                                          s_wide);
+    }
+}
+
 // Cells the main kernel deferred, with register-resident lists. LEVEL 0 (list 0: merged lists of
 // 5-6 entries, more than 4 own entries, or 2-4 entries next to a grain below its reference
 // volume) computes 2-6 entries with capacity 8; LEVEL 1 (list 1) computes 7 and 8; LEVEL 2
@@ -608,13 +640,17 @@ __global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A)
 // to the 16-entry kernel. Splitting the sizes keeps the register count of the common level low
 // and the few long lists off the private-memory path.
 template <typename TV, typename TC, bool GRAD, bool DG, int LEVEL>
-__global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs A)
+__global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restrict__ batch, int32_t ndom)
 {
+    // gridDim.x / ndom blocks per domain, each domain's list walked grid-stride by its blocks
+    const unsigned per = gridDim.x / static_cast<unsigned>(ndom);
+    const unsigned dom = blockIdx.x / per, lblock = blockIdx.x - dom * per;
+    const StepArgs& A = batch[dom];
     constexpr int K = LEVEL < 2 ? 8 : 12;
     const unsigned n = A.work_count[LEVEL];
     const uint32_t* list = A.work_list + LEVEL * A.ncells;
-    const unsigned stride = gridDim.x * kWideBlock;
-    for (unsigned base = blockIdx.x * kWideBlock; base < n; base += stride) {
+    const unsigned stride = per * kWideBlock;
+    for (unsigned base = lblock * kWideBlock; base < n; base += stride) {
         const unsigned i = base + threadIdx.x;
         const bool valid = i < n;
         CellList<K, TC, GRAD> L;
@@ -666,12 +702,15 @@ __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs A)
 
 // Cells whose merged list is longer than 8 entries (up to 16, held in private memory).
 template <typename TV, typename TC, bool GRAD, bool DG>
-__global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs A)
+__global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs* __restrict__ batch, int32_t ndom)
 {
     constexpr int K = kWideSlots;
+    const unsigned per = gridDim.x / static_cast<unsigned>(ndom);
+    const unsigned dom = blockIdx.x / per, lblock = blockIdx.x - dom * per;
+    const StepArgs& A = batch[dom];
     const unsigned n = A.work_count[3];
-    const unsigned stride = gridDim.x * kWideBlock;
-    for (unsigned base = blockIdx.x * kWideBlock; base < n; base += stride) {
+    const unsigned stride = per * kWideBlock;
+    for (unsigned base = lblock * kWideBlock; base < n; base += stride) {
         const unsigned i = base + threadIdx.x;
         const bool valid = i < n;
         CellList<K, TC, GRAD> L;
@@ -781,9 +820,11 @@ __device__ double limbs_to_double(const unsigned long long l[kLimbs])
 
 // PhaseField::CalculateGrainsVolume tail (src/PhaseField.cpp:893-951): MAXVolume, VolumeRatio,
 // Exist/Stage; clears the accumulators and the deferred-cell counter for the next step.
-__global__ void grains_kernel(const GrainArgs G)
+__global__ void grains_kernel(const GrainArgs* __restrict__ batch, const uint32_t* __restrict__ first, int32_t ndom)
 {
-    const int g = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int dom = ndom > 1 ? batch_domain(first, ndom, blockIdx.x) : 0;
+    const GrainArgs& G = batch[dom];
+    const int g = static_cast<int>(blockIdx.x - first[dom]) * blockDim.x + threadIdx.x;
     if (g == 0) {
         G.stats[0] += G.work_count[0];
         G.stats[1] += G.work_count[1];
@@ -841,52 +882,40 @@ int launch_copy(const void* src, void* dst, int64_t bytes, void* stream)
 // Two variants per precision: the plain isotropic step, and the full one (orientation-dependent
 // energies and constant driving force; with ISO tables or no driving-force pairs it reduces to
 // the plain arithmetic at run time).
-int launch_step(const StepArgs& a, bool fp32, bool full, void* stream)
+int launch_step(const StepArgs& one, const StepArgs* args, const uint32_t* first, int ndom, unsigned blocks, bool fp32,
+                bool full, bool uni, void* stream)
 {
-    const dim3 grid(static_cast<unsigned>((a.ncells + kBlock - 1) / kBlock));
+    const dim3 grid(blocks);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const bool uni = a.uniform != 0;
+#define MPF_STEP(TV, G, D, U) hipLaunchKernelGGL((step_kernel<TV, TV, G, D, U>), grid, dim3(kBlock), 0, s, one, args, first, ndom)
     if (fp32) {
-        if (full)     hipLaunchKernelGGL((step_kernel<float, float, true, true, false>), grid, dim3(kBlock), 0, s, a);
-        else if (uni) hipLaunchKernelGGL((step_kernel<float, float, false, false, true>), grid, dim3(kBlock), 0, s, a);
-        else          hipLaunchKernelGGL((step_kernel<float, float, false, false, false>), grid, dim3(kBlock), 0, s, a);
+        if (full)     MPF_STEP(float, true, true, false);
+        else if (uni) MPF_STEP(float, false, false, true);
+        else          MPF_STEP(float, false, false, false);
     } else {
-        if (full)     hipLaunchKernelGGL((step_kernel<double, double, true, true, false>), grid, dim3(kBlock), 0, s, a);
-        else if (uni) hipLaunchKernelGGL((step_kernel<double, double, false, false, true>), grid, dim3(kBlock), 0, s, a);
-        else          hipLaunchKernelGGL((step_kernel<double, double, false, false, false>), grid, dim3(kBlock), 0, s, a);
+        if (full)     MPF_STEP(double, true, true, false);
+        else if (uni) MPF_STEP(double, false, false, true);
+        else          MPF_STEP(double, false, false, false);
     }
+#undef MPF_STEP
     return static_cast<int>(hipGetLastError());
 }
 
-int launch_deferred(const StepArgs& a, bool fp32, bool full, void* stream)
+int launch_deferred(const StepArgs* args, int ndom, int per_domain, bool fp32, bool full, void* stream)
 {
-    const dim3 grid(static_cast<unsigned>(a.deferred_grid > 0 ? a.deferred_grid : kWideGrid));
+    const dim3 grid(static_cast<unsigned>(per_domain) * static_cast<unsigned>(ndom));
     hipStream_t s = static_cast<hipStream_t>(stream);
+#define MPF_DEFERRED(TV, F)                                                                              \
+    hipLaunchKernelGGL((mid_kernel<TV, TV, F, F, 0>), grid, dim3(kWideBlock), 0, s, args, ndom);    \
+    hipLaunchKernelGGL((mid_kernel<TV, TV, F, F, 1>), grid, dim3(kWideBlock), 0, s, args, ndom);    \
+    hipLaunchKernelGGL((mid_kernel<TV, TV, F, F, 2>), grid, dim3(kWideBlock), 0, s, args, ndom);    \
+    hipLaunchKernelGGL((wide_kernel<TV, TV, F, F>), grid, dim3(kWideBlock), 0, s, args, ndom);
     if (fp32) {
-        if (full) {
-            hipLaunchKernelGGL((mid_kernel<float, float, true, true, 0>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<float, float, true, true, 1>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<float, float, true, true, 2>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((wide_kernel<float, float, true, true>), grid, dim3(kWideBlock), 0, s, a);
-        } else {
-            hipLaunchKernelGGL((mid_kernel<float, float, false, false, 0>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<float, float, false, false, 1>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<float, float, false, false, 2>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((wide_kernel<float, float, false, false>), grid, dim3(kWideBlock), 0, s, a);
-        }
+        if (full) { MPF_DEFERRED(float, true) } else { MPF_DEFERRED(float, false) }
     } else {
-        if (full) {
-            hipLaunchKernelGGL((mid_kernel<double, double, true, true, 0>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<double, double, true, true, 1>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<double, double, true, true, 2>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((wide_kernel<double, double, true, true>), grid, dim3(kWideBlock), 0, s, a);
-        } else {
-            hipLaunchKernelGGL((mid_kernel<double, double, false, false, 0>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<double, double, false, false, 1>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((mid_kernel<double, double, false, false, 2>), grid, dim3(kWideBlock), 0, s, a);
-            hipLaunchKernelGGL((wide_kernel<double, double, false, false>), grid, dim3(kWideBlock), 0, s, a);
-        }
+        if (full) { MPF_DEFERRED(double, true) } else { MPF_DEFERRED(double, false) }
     }
+#undef MPF_DEFERRED
     return static_cast<int>(hipGetLastError());
 }
 
@@ -899,11 +928,9 @@ int launch_finalize_init(const StepArgs& a, bool fp32, void* stream)
     return static_cast<int>(hipGetLastError());
 }
 
-int launch_grains(const GrainArgs& g, void* stream)
+int launch_grains(const GrainArgs* args, const uint32_t* first, int ndom, unsigned blocks, void* stream)
 {
-    const int n = g.ngrains > 0 ? g.ngrains : 1;
-    const dim3 grid(static_cast<unsigned>((n + 255) / 256));
-    hipLaunchKernelGGL(grains_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g);
+    hipLaunchKernelGGL(grains_kernel, dim3(blocks), dim3(kGrainBlock), 0, static_cast<hipStream_t>(stream), args, first, ndom);
     return static_cast<int>(hipGetLastError());
 }
 
