@@ -56,6 +56,12 @@ typedef enum mpf_energy_model {    /* InterfaceEnergyModels subset, InterfaceEne
     MPF_ENERGY_CUBIC = 1
 } mpf_energy_model;
 
+typedef enum mpf_dg_weights {      /* AveragingWeightsModes, include/DrivingForce.h:46-51 */
+    MPF_DG_WEIGHTS_PHASEFIELDS = 0,
+    MPF_DG_WEIGHTS_RANGE = 1,
+    MPF_DG_WEIGHTS_COUNTER = 2
+} mpf_dg_weights;
+
 /* GridParameters (include/GridParameters.h:37-172) + BoundaryConditions + storage options. */
 typedef struct mpf_grid {
     int32_t nx, ny, nz;            /* cells per axis; 0 = reduced axis (GridParameters.cpp:645-706) */
@@ -83,6 +89,12 @@ typedef struct mpf_params {
     const double* dg_value;        /* UDF_Value_n_m, antisymmetric [J/m^3] */
     double dg_limit;               /* DrivingForce Limit_n_m (default 0.95) */
     double regularization_factor;  /* InterfaceProperties RegularizationFactor (default 1.0) */
+    /* DrivingForce::Average (src/DrivingForce.cpp:242-255, 293-478), read by ReadInput (:126-152).
+     * The reference's default is Average = Yes; 0 here selects SkipAverage (:257-274). */
+    int32_t dg_average;            /* Average: 1 = CollectAverage + DistributeAverage */
+    int32_t dg_range;              /* Range in cells; <= 0: (int)iwidth */
+    double dg_threshold;           /* Threshold (PhiThreshold); <= 0: iwidth < 5 ? iwidth/15 : 1/3 */
+    int32_t dg_weights;            /* WeightsMode (mpf_dg_weights) */
 } mpf_params;
 
 typedef struct mpf_ctx mpf_ctx;

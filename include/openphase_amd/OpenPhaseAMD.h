@@ -441,17 +441,27 @@ class UserDrivingForce {
 
 class DrivingForce {
  public:
-    DrivingForce(Settings& s, const std::string& file) : Nphases(s.Nphases)
+    DrivingForce(Settings& s, const std::string& file) : Nphases(s.Nphases) { Read(InputFile(file), s.Grid.iWidth); }
+    // DrivingForce::Initialize + ReadInput (src/DrivingForce.cpp:45-90, 126-152): Average (default
+    // Yes), Range (IWidth), Threshold (1/3 for IWidth >= 5, else IWidth/15), WeightsMode, Limit
+    void Read(const InputFile& in, double iwidth)
     {
-        InputFile in(file);
-        if (in.getb("DrivingForce", "Average", true))
-            throw std::runtime_error("DrivingForce averaging is not implemented: set $Average : No");
+        Averaging = in.getb("DrivingForce", "Average", true);
+        Range = in.geti("DrivingForce", "Range", static_cast<int>(iwidth));
+        PhiThreshold = in.getd("DrivingForce", "Threshold", iwidth < 5.0 ? iwidth / 15.0 : 1.0 / 3.0);
+        const std::string w = in.get("DrivingForce", "WeightsMode", "PHASEFIELDS");
+        WeightsMode = w == "RANGE" ? MPF_DG_WEIGHTS_RANGE : (w == "COUNTER" ? MPF_DG_WEIGHTS_COUNTER : MPF_DG_WEIGHTS_PHASEFIELDS);
         Limit = in.getd("DrivingForce", "Limit_0_0", 0.95);
     }
+    // The averaging and the merge run inside the fused step of PhaseField::MergeIncrements.
     void Clear() {}
     void Average(const PhaseField&, const BoundaryConditions&) {}
     void MergePhaseFieldIncrements(PhaseField&, InterfaceProperties&) {}
     size_t Nphases;
+    bool Averaging = true;
+    int Range = 5;
+    double PhiThreshold = 1.0 / 3.0;
+    int WeightsMode = MPF_DG_WEIGHTS_PHASEFIELDS;
     double Limit = 0.95;
 };
 
@@ -508,6 +518,14 @@ inline void PhaseField::ensure_context()
         p.dg_mode = udf ? udf->Mode.data() : nullptr;
         p.dg_value = udf ? udf->Value.data() : nullptr;
         p.dg_limit = 0.95;
+        if (udf) {
+            DrivingForce df(settings_, file_);
+            p.dg_limit = df.Limit;
+            p.dg_average = df.Averaging ? 1 : 0;
+            p.dg_range = df.Range;
+            p.dg_threshold = df.PhiThreshold;
+            p.dg_weights = df.WeightsMode;
+        }
         p.regularization_factor = ip.RegularizationFactor;
         check(mpf_create(&g, &p, &ctx_), nullptr, "mpf_create");
         driving_force_ = static_cast<bool>(udf);

@@ -20,6 +20,7 @@ STATUS_NAMES = {
 MPF_FP64, MPF_FP32 = 0, 1
 MPF_TJ_MODEL, MPF_TJ_VALUE, MPF_TJ_NONE = 0, 1, 2
 MPF_ENERGY_ISO, MPF_ENERGY_CUBIC = 0, 1
+MPF_DG_WEIGHTS_PHASEFIELDS, MPF_DG_WEIGHTS_RANGE, MPF_DG_WEIGHTS_COUNTER = 0, 1, 2
 
 # Every symbol the header declares (checked by the CPU tests without calling into the GPU).
 EXPORTED = [
@@ -51,6 +52,8 @@ class MpfParams(C.Structure):
         ("consider_nucleus_volume", C.c_int32), ("nucleus_volume_factor", C.c_double),
         ("dg_mode", C.POINTER(C.c_int32)), ("dg_value", C.POINTER(C.c_double)),
         ("dg_limit", C.c_double), ("regularization_factor", C.c_double),
+        ("dg_average", C.c_int32), ("dg_range", C.c_int32), ("dg_threshold", C.c_double),
+        ("dg_weights", C.c_int32),
     ]
 
 

@@ -42,7 +42,7 @@ def build_hip() -> Path:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
-    srcs = [CSRC / "mpf_kernels.hip", CSRC / "mpf_adjoint.hip", CSRC / "mpf_capi.cpp"]
+    srcs = [CSRC / "mpf_kernels.hip", CSRC / "mpf_adjoint.hip", CSRC / "mpf_dgavg.hip", CSRC / "mpf_capi.cpp"]
     compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
     procs = []
     objs = []

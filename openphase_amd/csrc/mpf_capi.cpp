@@ -58,6 +58,12 @@ struct mpf_ctx {
     double nucleus_volume_factor = 1.0, regularization = 1.0, dg_limit = 0.0;
     double wl[4] = {0, 0, 0, 0}, wg[4] = {0, 0, 0, 0};
     double prefactor = 0.0, dg_prefactor = 0.0;
+    // driving-force averaging (DrivingForce::Average); tmp and averaged value per pair of own entries
+    bool dg_avg = false;
+    int avg_range = 0, avg_weights = 0, avg_xr = 0, avg_yr = 0, avg_zr = 0;
+    double avg_threshold = 0.0;
+    double* d_avg_tmp = nullptr;
+    double* d_avg = nullptr;
 
     hipStream_t stream = nullptr;
     uint8_t* cnt[2] = {nullptr, nullptr};
@@ -188,6 +194,7 @@ mpf::StepArgs step_args(const mpf_ctx* c, double dt, int src)
     a.mu0 = c->mu[0];
     a.sigma = c->d_sigma; a.mu = c->d_mu; a.eps = c->d_eps; a.max_energy = c->d_maxe;
     a.emodel = c->d_emodel; a.dg_mode = c->d_dgmode; a.dg_value = c->d_dgval; a.dg_allowed = c->d_dgallowed;
+    a.dg_avg = c->dg_avg ? c->d_avg : nullptr;
     a.gphase = c->g_phase; a.gvr = c->g_vr; a.grot = c->g_rot; a.gsolid = c->g_solid;
     a.cnt_in = c->cnt[src]; a.id_in = c->id[src]; a.val_in = c->val[src];
     a.cnt_out = c->cnt[1 - src]; a.id_out = c->id[1 - src]; a.val_out = c->val[1 - src];
@@ -251,9 +258,27 @@ mpf_status sync_args(mpf_ctx* c, double dt, hipStream_t stream)
     return c->hip(e, "argument upload");
 }
 
+mpf::AvgArgs avg_args(const mpf_ctx* c, const mpf::StepArgs& s)
+{
+    mpf::AvgArgs a{};
+    a.s = s;
+    a.tmp = c->d_avg_tmp;
+    a.avg = c->d_avg;
+    a.range = c->avg_range;
+    a.xr = c->avg_xr; a.yr = c->avg_yr; a.zr = c->avg_zr;
+    a.weights = c->avg_weights;
+    a.threshold = c->avg_threshold;
+    return a;
+}
+
 mpf_status launch_step_part(mpf_ctx* c, hipStream_t stream)
 {
     const bool full = c->grad || c->dg;
+    if (c->dg_avg) {
+        mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_dg_average(avg_args(c, c->h_args[c->cur]), c->fp32, stream)),
+                               "driving-force averaging kernels");
+        if (st) return st;
+    }
     return c->hip(static_cast<hipError_t>(mpf::launch_step(c->h_args[c->cur], nullptr, nullptr, 1, step_blocks(c), c->fp32,
                                                            full, c->uniform, stream)), "step kernel");
 }
@@ -381,6 +406,23 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
                 c->dgallowed[k] = allowed;
             }
     }
+    // DrivingForce::Initialize / ReadInput (src/DrivingForce.cpp:45-90, 126-152)
+    c->dg_avg = c->dg && params->dg_average != 0;
+    if (c->dg_avg) {
+        c->avg_range = params->dg_range > 0 ? params->dg_range : static_cast<int>(grid->iwidth);
+        const double thr = params->dg_threshold > 0.0 ? params->dg_threshold
+                                                      : (grid->iwidth < 5.0 ? grid->iwidth / 15.0 : 1.0 / 3.0);
+        c->avg_threshold = std::sqrt(thr * (1.0 - thr));
+        c->avg_weights = params->dg_weights;
+        if (c->avg_weights < MPF_DG_WEIGHTS_PHASEFIELDS || c->avg_weights > MPF_DG_WEIGHTS_COUNTER)
+            return bail(MPF_ERR_ARG, "unknown driving-force weights mode");
+        // the reference's periodic halo of Range cells is a wrapped index only on axes of >= Range cells
+        if ((c->dnx && c->nx < c->avg_range) || (c->dny && c->ny < c->avg_range) || (c->dnz && c->nz < c->avg_range))
+            return bail(MPF_ERR_UNSUPPORTED, "driving-force averaging needs every active axis >= Range cells");
+        c->avg_xr = std::min(c->avg_range, c->nx - 1) * c->dnx;
+        c->avg_yr = std::min(c->avg_range, c->ny - 1) * c->dny;
+        c->avg_zr = std::min(c->avg_range, c->nz - 1) * c->dnz;
+    }
 
     // Stencil weights by number of non-zero offset components: Laplacian / dx^2, gradient / dx
     // (LaplacianStencil.h:85-119, GradientStencil.h:58-92; 1D/2D use the middle slices).
@@ -418,6 +460,11 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         c->hip(dalloc(&c->d_work_list, 4 * N), "work list") || c->hip(dalloc(&c->d_work_count, 4), "work list") ||
         c->hip(dalloc(&c->d_stats, 2), "statistics"))
         return bail(MPF_ERR_NOMEM, c->err.c_str());
+    if (c->dg_avg) {
+        const size_t pairs = static_cast<size_t>(c->slots) * (c->slots - 1) / 2;
+        if (c->hip(dalloc(&c->d_avg_tmp, pairs * N), "averaging arrays") || c->hip(dalloc(&c->d_avg, pairs * N), "averaging arrays"))
+            return bail(MPF_ERR_NOMEM, c->err.c_str());
+    }
     hipError_t e = hipSuccess;
     h2d(e, c->d_sigma, c->sigma.data(), np2);
     h2d(e, c->d_mu, c->mu.data(), np2);
@@ -808,6 +855,7 @@ void mpf_destroy(mpf_ctx* c)
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
     dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats);
+    dfree(c->d_avg_tmp); dfree(c->d_avg);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -929,6 +977,7 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
     if (!c || !lam_next || !lam_prev || !grad) return MPF_ERR_ARG;
     if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
     if (c->fp32) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint runs on FP64 states");
+    if (c->dg_avg) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint takes the driving force without averaging");
     if (c->nphases > 4) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint handles at most 4 phases");
     const size_t N = static_cast<size_t>(c->ncells);
     constexpr size_t KA = mpf::kAdjSlots;
@@ -1103,7 +1152,11 @@ mpf_status capture(mpf_ensemble* e)
             const int n = static_cast<int>(g.members.size());
             mpf::StepArgs* args = half == 0 ? g.d_a1 : g.d_a2;
             const unsigned blocks = g.h_first[n], gblocks = g.h_gfirst[n];
-            rc = mpf::launch_step(half == 0 ? g.h_a1[0] : g.h_a2[0], args, g.d_first, n, blocks, g.fp32, g.full, g.uni,
+            for (int k = 0; k < n && rc == 0; ++k) {
+                const mpf_ctx* c = e->members[g.members[k]];
+                if (c->dg_avg) rc = mpf::launch_dg_average(avg_args(c, half == 0 ? g.h_a1[k] : g.h_a2[k]), g.fp32, e->stream);
+            }
+            if (!rc) rc = mpf::launch_step(half == 0 ? g.h_a1[0] : g.h_a2[0], args, g.d_first, n, blocks, g.fp32, g.full, g.uni,
                                   e->stream);
             if (!rc) rc = mpf::launch_deferred(args, n, g.per_domain, g.fp32, g.full, e->stream);
             if (!rc) rc = mpf::launch_grains(g.d_g, g.d_gfirst, n, gblocks, e->stream);

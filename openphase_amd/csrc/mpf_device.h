@@ -43,6 +43,10 @@ struct StepArgs {
     const int32_t* dg_mode;      // nphases^2, 1 = constant driving force for the pair
     const double* dg_value;      // nphases^2 antisymmetric driving force
     const double* dg_allowed;    // nphases^2 tanh limiter amplitude
+    // driving force averaged over the interface for this step (DrivingForce::Average), one value
+    // per pair of own entries a < b at plane b(b-1)/2 + a, [pairs][ncells]; null: the phase-pair
+    // value (SkipAverage)
+    const double* dg_avg;
     const int32_t* gphase;       // per grain: thermodynamic phase
     const double* gvr;           // per grain: VolumeRatio
     const double* grot;          // per grain: rotation matrix, row-major
@@ -99,6 +103,20 @@ constexpr int kAdjSlots = 8;     // merged entries per cell the adjoint handles
 int launch_adjoint(const AdjointArgs& a, void* stream);
 
 constexpr int kGrainBlock = 256;                     // threads per block of the grain kernel
+
+// Driving-force averaging (mpf_dgavg.hip): CollectAverage into `tmp`, DistributeAverage into
+// `avg` (= s.dg_avg), both [pairs][ncells] with pair a < b of the own entries at b(b-1)/2 + a.
+struct AvgArgs {
+    StepArgs s;                  // state of the step start (cnt_in, id_in, val_in), grains, tables
+    double* tmp;
+    double* avg;
+    int32_t range;               // Range (sphere radius, cells)
+    int32_t xr, yr, zr;          // offset bound per axis: min(Range, N - 1) on active axes, else 0
+    int32_t weights;             // 0 PHASEFIELDS, 1 RANGE, 2 COUNTER
+    double threshold;            // sqrt(PhiThreshold * (1 - PhiThreshold))
+};
+constexpr int kAvgBlock = 256;
+int launch_dg_average(const AvgArgs& a, bool fp32, void* stream);
 
 // Host-side launchers (mpf_kernels.hip); return a hipError_t value. The step, deferred-cell and
 // grain kernels run over one or several domains of the same kernel variant in one launch: `args`

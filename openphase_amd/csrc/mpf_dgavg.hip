@@ -1,0 +1,156 @@
+// MI355X (gfx950) kernels of the driving-force averaging over the interface
+// (DrivingForce::Average with Averaging on, src/DrivingForce.cpp:242-255).
+//
+// The reference keeps a Force entry per VALUE-mode phase-field pair of every interface node
+// (UserDrivingForce::SetDrivingForce, src/UserDrivingForce.cpp:152-175) with raw = Value(phaseA,
+// phaseB) and weight = sqrt(phiA phiB) (SetWeights, :276-291). CollectAverage (:293-379) replaces
+// raw by its weighted mean over the sphere of radius Range (tmp); DistributeAverage (:381-478)
+// averages tmp again (average), which MergePhaseFieldIncrementsSR then limits and merges. Here no
+// Force storage exists: an entry's raw value and weight follow from the node's own list (grains,
+// values) and the phase-pair table, so both passes read the step-start state directly. One thread
+// per cell walks its own pairs and, per pair, the offsets x outer, z inner (the reference's loop
+// order, so every sum is formed in the same order); a neighbour contributes when it is an
+// interface cell holding both grains. tmp and the averaged value are stored per pair of own
+// entries (a < b at plane b(b-1)/2 + a) for the step kernel, which reads them in place of the
+// phase-pair value.
+//
+// Built with -ffp-contract=off like the step kernels: the FP64 sums are the reference's.
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "mpf_device.h"
+#include "mpf_lists.h"
+
+namespace mpf {
+
+namespace {
+
+__device__ __forceinline__ int pair_plane(int a, int b)  // a < b
+{
+    return b * (b - 1) / 2 + a;
+}
+
+// Value of own entry t of cell c (count code `code`). FP32 lists recover their implicit entry as
+// 1 - (sum of the others in slot order), as the step does.
+template <typename TV>
+__device__ double entry_value(const StepArgs& A, int64_t c, uint8_t code, int t)
+{
+    const int64_t N = A.ncells;
+    const int imp = ListCode<TV>::implicit(code);
+    if (t != imp) return static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(t) * N + c));
+    const int n = ListCode<TV>::count(code);
+    double others = 0.0;
+    for (int s = 0; s < n; ++s)
+        if (s != imp) others += static_cast<double>(load_val<TV>(A.val_in, static_cast<int64_t>(s) * N + c));
+    return 1.0 - others;
+}
+
+// NodeDF::get_weight of pair (ga, gb) at cell nb (include/Containers/NodeDF.h:401-410): a Force
+// entry exists on an interface cell (more than one own entry) whose list holds both grains (the
+// pair's phases give the same mode everywhere); entries of zero-valued appended grains weigh 0 and
+// never count. On success: the weight, the pair plane and whether ga precedes gb in that cell's
+// list (the entry's stored order; get_raw / get_tmp flip the sign otherwise).
+template <typename TV>
+__device__ bool find_pair(const StepArgs& A, int64_t nb, uint32_t ga, uint32_t gb, double& w, int& plane, bool& fwd)
+{
+    const int64_t N = A.ncells;
+    const uint8_t code = A.cnt_in[nb];
+    const int n = ListCode<TV>::count(code);
+    if (n < 2) return false;
+    int ia = -1, ib = -1;
+    for (int s = 0; s < n; ++s) {
+        const uint32_t g = A.id_in[static_cast<int64_t>(s) * N + nb];
+        ia = g == ga ? s : ia;
+        ib = g == gb ? s : ib;
+    }
+    if (ia < 0 || ib < 0) return false;
+    w = sqrt(entry_value<TV>(A, nb, code, ia) * entry_value<TV>(A, nb, code, ib));
+    fwd = ia < ib;
+    plane = fwd ? pair_plane(ia, ib) : pair_plane(ib, ia);
+    return true;
+}
+
+// SECOND = false: CollectAverage into tmp; true: DistributeAverage into avg.
+template <typename TV, bool SECOND>
+__global__ __launch_bounds__(kAvgBlock) void dg_average_kernel(const AvgArgs P)
+{
+    const StepArgs& A = P.s;
+    const int64_t N = A.ncells;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kAvgBlock + threadIdx.x;
+    if (c >= N) return;
+    const uint8_t code = A.cnt_in[c];
+    const int n = ListCode<TV>::count(code);
+    if (n < 2) return;  // Force entries live on interface cells only
+    const int64_t nyz = static_cast<int64_t>(A.ny) * A.nz;
+    const int x = static_cast<int>(c / nyz), y = static_cast<int>((c / A.nz) % A.ny), z = static_cast<int>(c % A.nz);
+    const int NP = A.nphases;
+    const double R = static_cast<double>(P.range);
+    for (int b = 1; b < n; ++b)
+        for (int a = 0; a < b; ++a) {
+            const uint32_t ga = A.id_in[static_cast<int64_t>(a) * N + c], gb = A.id_in[static_cast<int64_t>(b) * N + c];
+            const int pa = A.gphase[ga], pb = A.gphase[gb];
+            if (A.dg_mode[pa * NP + pb] != 1) continue;  // no Force entry for this pair
+            const int plane = pair_plane(a, b);
+            const double w = sqrt(entry_value<TV>(A, c, code, a) * entry_value<TV>(A, c, code, b));
+            const double scale = A.gvr[ga] * A.gvr[gb];
+            const double cut = P.threshold * scale;
+            const double vab = A.dg_value[pa * NP + pb], vba = A.dg_value[pb * NP + pa];
+            // own value the pass falls back to: raw (collect) or tmp (distribute)
+            const double own = SECOND ? P.tmp[static_cast<int64_t>(plane) * N + c] : vab;
+            double result = 0.0;  // a fresh entry's tmp / average
+            if (SECOND ? (w > 0.0) : (w > cut)) {
+                double value = 0.0, sum = 0.0;
+                int counter = 0;
+                for (int ii = -P.xr; ii <= P.xr; ii++)
+                    for (int jj = -P.yr; jj <= P.yr; jj++)
+                        for (int kk = -P.zr; kk <= P.zr; kk++) {
+                            const double wdist = R - sqrt(static_cast<double>(ii * ii + jj * jj + kk * kk));
+                            if (!(wdist > 0.0)) continue;
+                            const int64_t nb = (static_cast<int64_t>(wrap1(x + ii, A.nx)) * A.ny + wrap1(y + jj, A.ny)) * A.nz +
+                                               wrap1(z + kk, A.nz);
+                            double wphi = 0.0;
+                            int np = 0;
+                            bool fwd = true;
+                            if (!find_pair<TV>(A, nb, ga, gb, wphi, np, fwd)) continue;
+                            if (!(SECOND ? (wphi > cut) : (wphi > 0.0))) continue;
+                            double wt = 0.0;
+                            if (P.weights == 1) { wt = wdist; sum += wt; }
+                            else if (P.weights == 0) { wt = wphi; sum += wt; }
+                            else { counter++; wt = 1.0; }
+                            double v;
+                            if (SECOND) {
+                                const double t = P.tmp[static_cast<int64_t>(np) * N + nb];
+                                v = fwd ? t : -t;
+                            } else {
+                                v = fwd ? vab : -vba;
+                            }
+                            value += wt * v;
+                        }
+                if (P.weights == 2) result = counter != 0 ? value / static_cast<double>(counter) : own;
+                else result = sum > 0.0 ? value / sum : own;
+            }
+            // DistributeAverage's weight-0 branch (Seed grains) cannot occur: own entries are
+            // above the finalize cutoff and no grain is a Seed
+            (SECOND ? P.avg : P.tmp)[static_cast<int64_t>(plane) * N + c] = result;
+        }
+}
+
+}  // namespace
+
+int launch_dg_average(const AvgArgs& a, bool fp32, void* stream)
+{
+    const dim3 grid(static_cast<unsigned>((a.s.ncells + kAvgBlock - 1) / kAvgBlock));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (fp32) {
+        hipLaunchKernelGGL((dg_average_kernel<float, false>), grid, dim3(kAvgBlock), 0, s, a);
+        hipLaunchKernelGGL((dg_average_kernel<float, true>), grid, dim3(kAvgBlock), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((dg_average_kernel<double, false>), grid, dim3(kAvgBlock), 0, s, a);
+        hipLaunchKernelGGL((dg_average_kernel<double, true>), grid, dim3(kAvgBlock), 0, s, a);
+    }
+    return static_cast<int>(hipGetLastError());
+}
+
+}  // namespace mpf

@@ -37,13 +37,24 @@ namespace mpf {
 
 namespace {
 
+// Driving force of pair (a, b) before the limiter: the value averaged over the interface for this
+// step when averaging is on (own entries only: a pair with a zero-valued appended entry weighs 0),
+// else the phase-pair value (DrivingForce::SkipAverage).
+template <typename TC>
+__device__ __forceinline__ TC dg_local(const StepArgs& A, int a, int b, int n0, int64_t cell, int k)
+{
+    if (A.dg_avg != nullptr && b < n0)
+        return static_cast<TC>(A.dg_avg[static_cast<int64_t>(b * (b - 1) / 2 + a) * A.ncells + cell]);
+    return static_cast<TC>(A.dg_value[k]);
+}
+
 // IP.Set + increments + normalize + merge + finalize of a wide cell whose merged list has C
 // entries (EXACT) or at most C entries (ne at run time). Only entries < C are touched.
 // PATH selects the increment formula at compile time: 0 both (chosen per cell), 1 only the
 // equal-energy form (returns false, doing nothing, for a cell that needs the general one), 2 only
 // the general form (exact for every cell: the equal-energy form drops exact zero terms).
 template <int C, bool EXACT, int K, typename TC, bool GRAD, bool DG, int PATH = 0>
-__device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, GRAD>& L, bool& empty)
+__device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, GRAD>& L, bool& empty, int64_t cell)
 {
     constexpr int U = C <= kFastSlots ? C : 1;
     constexpr int NPC = C * (C - 1) / 2;
@@ -94,7 +105,7 @@ __device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, 
             TC d = s0 * (X[a] - X[b]);
             d *= mf;
             if (DG && n0 > 1 && A.dg_mode[ph[a] * NP + ph[b]] == 1) {
-                const TC raw = static_cast<TC>(A.dg_value[ph[a] * NP + ph[b]]);
+                const TC raw = dg_local<TC>(A, a, b, n0, cell, ph[a] * NP + ph[b]);
                 const TC weight = sqrt(L.ev(a) * L.ev(b));
                 const TC allowed = static_cast<TC>(A.dg_allowed[ph[a] * NP + ph[b]]);
                 const TC dgl = allowed * tanh(raw / allowed);
@@ -175,7 +186,7 @@ __device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, 
             d *= mob * static_cast<TC>(norm_1) * scale;
             // constant driving force on interface cells (flag 2 = more than one own entry)
             if (DG && n0 > 1 && A.dg_mode[ph[a] * NP + ph[b]] == 1) {
-                const TC raw = static_cast<TC>(A.dg_value[ph[a] * NP + ph[b]]);
+                const TC raw = dg_local<TC>(A, a, b, n0, cell, ph[a] * NP + ph[b]);
                 const TC weight = sqrt(L.ev(a) * L.ev(b));
                 const TC allowed = static_cast<TC>(A.dg_allowed[ph[a] * NP + ph[b]]);
                 const TC dgl = allowed * tanh(raw / allowed);
@@ -584,9 +595,9 @@ __device__ __forceinline__ void step_block(const StepArgs& A, int64_t base, uint
             constexpr int PATH = UNI ? 1 : 2;
             bool done = false;
             switch (L.ne) {
-            case 2: done = compute_cell<2, true, K, TC, GRAD, DG, PATH>(A, L, empty); break;
-            case 3: done = compute_cell<3, true, K, TC, GRAD, DG, PATH>(A, L, empty); break;
-            case 4: done = compute_cell<4, true, K, TC, GRAD, DG, PATH>(A, L, empty); break;
+            case 2: done = compute_cell<2, true, K, TC, GRAD, DG, PATH>(A, L, empty, cw); break;
+            case 3: done = compute_cell<3, true, K, TC, GRAD, DG, PATH>(A, L, empty, cw); break;
+            case 4: done = compute_cell<4, true, K, TC, GRAD, DG, PATH>(A, L, empty, cw); break;
             default: break;
             }
             if (!done) r = kCellDeferred;
@@ -663,23 +674,23 @@ __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restr
             if (r == kCellWide) {
                 if (LEVEL == 0) {
                     switch (L.ne) {
-                    case 2: compute_cell<2, true, K, TC, GRAD, DG>(A, L, empty); break;
-                    case 3: compute_cell<3, true, K, TC, GRAD, DG>(A, L, empty); break;
-                    case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty); break;
-                    case 5: compute_cell<5, true, K, TC, GRAD, DG>(A, L, empty); break;
-                    case 6: compute_cell<6, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 2: compute_cell<2, true, K, TC, GRAD, DG>(A, L, empty, c); break;
+                    case 3: compute_cell<3, true, K, TC, GRAD, DG>(A, L, empty, c); break;
+                    case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty, c); break;
+                    case 5: compute_cell<5, true, K, TC, GRAD, DG>(A, L, empty, c); break;
+                    case 6: compute_cell<6, true, K, TC, GRAD, DG>(A, L, empty, c); break;
                     default: r = kCellDeferred; break;
                     }
                 } else if (LEVEL == 1) {
                     switch (L.ne) {
-                    case 7: compute_cell<7, true, K, TC, GRAD, DG>(A, L, empty); break;
-                    case 8: compute_cell<8, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 7: compute_cell<7, true, K, TC, GRAD, DG>(A, L, empty, c); break;
+                    case 8: compute_cell<8, true, K, TC, GRAD, DG>(A, L, empty, c); break;
                     default: r = kCellDeferred; break;
                     }
                 } else {
                     switch (L.ne) {
-                    case 9: compute_cell<9, true, K, TC, GRAD, DG>(A, L, empty); break;
-                    case 10: compute_cell<10, true, K, TC, GRAD, DG>(A, L, empty); break;
+                    case 9: compute_cell<9, true, K, TC, GRAD, DG>(A, L, empty, c); break;
+                    case 10: compute_cell<10, true, K, TC, GRAD, DG>(A, L, empty, c); break;
                     default: r = kCellDeferred; break;
                     }
                 }
@@ -719,7 +730,7 @@ __global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs* __rest
         if (valid) {
             const int64_t c = A.work_list[3 * A.ncells + i];
             const int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
-            if (r == kCellWide) compute_cell<K, false, K, TC, GRAD, DG>(A, L, empty);
+            if (r == kCellWide) compute_cell<K, false, K, TC, GRAD, DG>(A, L, empty, c);
             if (r == kCellDeferred || !store_cell<K, TV, TC, GRAD>(A, c, L)) report_overflow<K, TC, GRAD>(A, c, L);
             if (empty) atomicOr(&A.status[kStatusEmpty], 1);
         }

@@ -38,6 +38,12 @@ class StepConfig:
     dg_mode: np.ndarray | None = None  # 1 = VALUE per phase pair
     dg_value: np.ndarray | None = None
     dg_limit: float = 0.95
+    # DrivingForce Average / Range / Threshold / WeightsMode (0 PHASEFIELDS, 1 RANGE, 2 COUNTER);
+    # Range <= 0 and Threshold <= 0 select the reference defaults (IWidth; 1/3 for IWidth >= 5)
+    dg_average: bool = False
+    dg_range: int = 0
+    dg_threshold: float = 0.0
+    dg_weights: int = 0
     regularization_factor: float = 1.0
     precision: int = _capi.MPF_FP64
     slots: int = 8
@@ -101,6 +107,10 @@ class Engine:
         p.dg_mode = ptr(arr(cfg.dg_mode, np.int32), C.c_int32)
         p.dg_value = ptr(arr(cfg.dg_value, np.float64), C.c_double)
         p.dg_limit = cfg.dg_limit
+        p.dg_average = int(cfg.dg_average)
+        p.dg_range = int(cfg.dg_range)
+        p.dg_threshold = float(cfg.dg_threshold)
+        p.dg_weights = int(cfg.dg_weights)
         p.regularization_factor = cfg.regularization_factor
         h = C.c_void_p()
         st = self.lib.mpf_create(C.byref(g), C.byref(p), C.byref(h))

@@ -41,6 +41,13 @@ typedef struct {
 
 #define ORC_PCAP (ORC_CAP * ORC_CAP)
 
+/* DrivingForceEntry (include/Containers/NodeDF.h:37-50): one per phase-field pair of an
+ * interface node, created by UserDrivingForce::SetDrivingForce in list order */
+typedef struct {
+    uint32_t a, b;
+    double raw, tmp, average, weight;
+} orc_force;
+
 typedef struct {
     int di, dj, dk;
     double w;          /* Laplacian weight / dx^2 */
@@ -82,6 +89,12 @@ struct orc_ctx {
     int max_final;  /* longest finalized list seen */
     int status;
     double max_energy, max_mobility; /* over phase pairs of the last IP.Set */
+    /* DrivingForce::Force of the current step (CSR over cells) and the averaging settings */
+    int avg_on, avg_range, avg_weights;
+    double avg_threshold;
+    orc_force* force;
+    long* force_off;
+    long force_cap;
 };
 
 static long cell_index(const orc_ctx* c, long i, long j, long k)
@@ -191,6 +204,21 @@ int orc_create(const orc_params* p, int32_t ngrains, const int32_t* grain_phase,
     c->dgval = p->dg_value ? dup_d(p->dg_value, np2, 0.0) : NULL;
     c->dgmode = (int32_t*)calloc(np2, sizeof(int32_t));
     for (size_t i = 0; i < np2 && c->dgmode; ++i) c->dgmode[i] = (p->dg_mode && p->dg_value) ? p->dg_mode[i] : 0;
+    /* DrivingForce::Initialize / ReadInput (src/DrivingForce.cpp:45-90, 126-152) */
+    c->avg_on = p->dg_average && p->dg_value && p->dg_mode;
+    c->avg_range = p->dg_range > 0 ? p->dg_range : (int)p->iwidth;
+    c->avg_threshold = p->dg_threshold > 0.0 ? p->dg_threshold : (p->iwidth < 5.0 ? p->iwidth / 15.0 : 1.0 / 3.0);
+    c->avg_weights = p->dg_weights;
+    if (c->avg_on) {
+        /* the periodic halo of Range cells equals a wrapped index only for axes >= Range */
+        if (c->avg_weights < 0 || c->avg_weights > 2 ||
+            (c->dn[0] && c->nx < c->avg_range) || (c->dn[1] && c->ny < c->avg_range) ||
+            (c->dn[2] && c->nz < c->avg_range)) {
+            free(c);
+            return ORC_ERR_ARG;
+        }
+        c->force_off = (long*)calloc((size_t)c->ncells + 1, sizeof(long));
+    }
     c->emodel = (int32_t*)calloc(np2, sizeof(int32_t));
     c->need_grad = 0;
     for (size_t i = 0; i < np2; ++i) {
@@ -209,7 +237,8 @@ int orc_create(const orc_params* p, int32_t ngrains, const int32_t* grain_phase,
     c->stage = (int*)calloc(ngrains ? ngrains : 1, sizeof(int));
     c->nodes = (orc_node*)calloc((size_t)c->ncells, sizeof(orc_node));
     c->tmp = (orc_node*)calloc((size_t)c->ncells, sizeof(orc_node));
-    if (!c->sigma || !c->mu || !c->eps || !c->emodel || !c->gphase || !c->grot || !c->nodes || !c->tmp) {
+    if (!c->sigma || !c->mu || !c->eps || !c->emodel || !c->gphase || !c->grot || !c->nodes || !c->tmp ||
+        (c->avg_on && !c->force_off)) {
         orc_destroy(c);
         return ORC_ERR_NOMEM;
     }
@@ -239,6 +268,7 @@ void orc_destroy(orc_ctx* c)
     free(c->volume); free(c->maxvolume); free(c->refvolume); free(c->vr);
     free(c->exist); free(c->stage);
     free(c->nodes); free(c->tmp);
+    free(c->force); free(c->force_off);
     free(c);
 }
 
@@ -470,15 +500,19 @@ static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
     }
 
     /* UserDrivingForce::SetDrivingForce (src/UserDrivingForce.cpp:152-175) on interface nodes,
-     * DrivingForce::SkipAverage (src/DrivingForce.cpp:257-274) and
-     * DrivingForce::MergePhaseFieldIncrementsSR (src/DrivingForce.cpp:497-602), Limiting on. */
+     * DrivingForce::SkipAverage (src/DrivingForce.cpp:257-274) or the averaged value of the
+     * node's Force entry (dg_average_pass), and DrivingForce::MergePhaseFieldIncrementsSR
+     * (src/DrivingForce.cpp:497-602), Limiting on. No grain is a Seed (grains enter Stable and
+     * only nucleation makes seeds), so the 1e-6 seed term of :528-536 never applies. */
     if (c->dgval && nd->flag == 2) {
         const double pref = ORC_PI / eta;
+        const orc_force* fe = c->avg_on ? &c->force[c->force_off[nd - c->nodes]] : NULL;
         for (int a = 0; a < n - 1; ++a)
         for (int b = a + 1; b < n; ++b) {
             int pa = c->gphase[nd->e[a].idx], pb = c->gphase[nd->e[b].idx];
             if (c->dgmode[pa * c->nphases + pb] != 1) continue; /* no Force entry for this pair */
             double raw = c->dgval[pa * c->nphases + pb];
+            if (fe) raw = (fe++)->average;
             double weight = sqrt(nd->e[a].val * nd->e[b].val);
             double allowed = c->p.dg_limit * pref * max_energy_of(c, pa, pb);
             allowed *= c->p.regularization_factor;
@@ -781,9 +815,129 @@ int orc_finalize_initialization(orc_ctx* c)
     return finalize_sr(c);
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Driving-force averaging over the interface                                                  */
+
+/* NodeDF::get_weight / get_raw / get_tmp (include/Containers/NodeDF.h:359-410): symmetric weight,
+ * antisymmetric raw and tmp (sign flips with the pair order), 0 for a missing pair */
+static const orc_force* force_find(const orc_ctx* c, long cell, uint32_t a, uint32_t b, double* sign)
+{
+    for (long q = c->force_off[cell]; q < c->force_off[cell + 1]; ++q) {
+        const orc_force* f = &c->force[q];
+        if (f->a == a && f->b == b) { *sign = 1.0; return f; }
+        if (f->a == b && f->b == a) { *sign = -1.0; return f; }
+    }
+    return NULL;
+}
+
+/* UserDrivingForce::SetDrivingForce (src/UserDrivingForce.cpp:152-175) after DrivingForce::Clear
+ * (src/DrivingForce.cpp:176-183): one entry per VALUE-mode pair of every interface node (the
+ * whole list, zero-valued entries included), raw = Value(phaseA, phaseB); then
+ * DrivingForce::SetWeights (:276-291): weight = sqrt(phiA*phiB). */
+static int dg_set_forces(orc_ctx* c)
+{
+    const long N = c->ncells;
+    const int np = c->nphases;
+    c->force_off[0] = 0;
+    for (long ci = 0; ci < N; ++ci) {
+        const orc_node* nd = &c->nodes[ci];
+        long m = 0;
+        if (nd->flag == 2)
+            for (int a = 0; a < nd->n - 1; ++a)
+            for (int b = a + 1; b < nd->n; ++b)
+                m += c->dgmode[c->gphase[nd->e[a].idx] * np + c->gphase[nd->e[b].idx]] == 1;
+        c->force_off[ci + 1] = c->force_off[ci] + m;
+    }
+    long total = c->force_off[N];
+    if (total > c->force_cap) {
+        orc_force* f = (orc_force*)realloc(c->force, (size_t)(total ? total : 1) * sizeof(orc_force));
+        if (!f) return ORC_ERR_NOMEM;
+        c->force = f;
+        c->force_cap = total;
+    }
+#pragma omp parallel for schedule(static)
+    for (long ci = 0; ci < N; ++ci) {
+        const orc_node* nd = &c->nodes[ci];
+        orc_force* f = &c->force[c->force_off[ci]];
+        if (nd->flag != 2) continue;
+        for (int a = 0; a < nd->n - 1; ++a)
+        for (int b = a + 1; b < nd->n; ++b) {
+            int pa = c->gphase[nd->e[a].idx], pb = c->gphase[nd->e[b].idx];
+            if (c->dgmode[pa * np + pb] != 1) continue;
+            f->a = nd->e[a].idx; f->b = nd->e[b].idx;
+            f->raw = c->dgval[pa * np + pb];
+            f->tmp = 0.0; f->average = 0.0;
+            f->weight = sqrt(nd->e[a].val * nd->e[b].val);
+            ++f;
+        }
+    }
+    return ORC_OK;
+}
+
+/* DrivingForce::CollectAverage (second == 0, src/DrivingForce.cpp:293-379) and
+ * DistributeAverage (second == 1, :381-478): weighted mean over the sphere of radius Range
+ * (offsets x outer, z inner, each clipped to N-1 on its axis) of the neighbours' raw (tmp)
+ * values, over neighbours whose weight exceeds 0 (threshold*VR_A*VR_B). The periodic
+ * BoundaryConditions copies of Force between the passes are wrapped indices here. */
+static void dg_average_pass(orc_ctx* c, int second)
+{
+    const int R = c->avg_range;
+    const int xr = (R < c->nx - 1 ? R : c->nx - 1) * c->dn[0];
+    const int yr = (R < c->ny - 1 ? R : c->ny - 1) * c->dn[1];
+    const int zr = (R < c->nz - 1 ? R : c->nz - 1) * c->dn[2];
+    const double thr = sqrt(c->avg_threshold * (1.0 - c->avg_threshold));
+    const long N = c->ncells;
+#pragma omp parallel for schedule(static)
+    for (long ci = 0; ci < N; ++ci) {
+        if (c->nodes[ci].flag != 2) continue;
+        long i = ci / ((long)c->ny * c->nz), j = (ci / c->nz) % c->ny, k = ci % c->nz;
+        for (long q = c->force_off[ci]; q < c->force_off[ci + 1]; ++q) {
+            orc_force* e = &c->force[q];
+            const double scale = c->vr[e->a] * c->vr[e->b];
+            if (second ? !(e->weight > 0.0) : !(e->weight > thr * scale)) {
+                if (second && (c->stage[e->a] == 0 || c->stage[e->b] == 0)) e->average = e->raw;
+                continue;
+            }
+            double value = 0.0, sum = 0.0;
+            int counter = 0;
+            for (int ii = -xr; ii <= xr; ii++)
+            for (int jj = -yr; jj <= yr; jj++)
+            for (int kk = -zr; kk <= zr; kk++) {
+                double wdist = R - sqrt((double)(ii * ii + jj * jj + kk * kk));
+                long nb = cell_index(c, wrap(i + ii, c->nx), wrap(j + jj, c->ny), wrap(k + kk, c->nz));
+                double sign = 1.0;
+                const orc_force* f = force_find(c, nb, e->a, e->b, &sign);
+                double wphi = f ? f->weight : 0.0;
+                if (wdist > 0.0 && (second ? wphi > thr * scale : wphi > 0.0)) {
+                    double w = 0.0;
+                    switch (c->avg_weights) {
+                    case 1: w = wdist; sum += w; break;
+                    case 0: w = wphi; sum += w; break;
+                    default: counter++; w = 1.0; break;
+                    }
+                    value += w * ((second ? f->tmp : f->raw) * sign);
+                }
+            }
+            double fallback = second ? e->tmp : e->raw;
+            double r;
+            if (c->avg_weights == 2) r = counter != 0 ? value / counter : fallback;
+            else r = sum > 0.0 ? value / sum : fallback;
+            if (second) e->average = r;
+            else e->tmp = r;
+        }
+    }
+}
+
 int orc_step(orc_ctx* c, double dt, int32_t nsteps)
 {
     for (int s = 0; s < nsteps; ++s) {
+        if (c->avg_on) {
+            /* DrivingForce::Average (src/DrivingForce.cpp:242-255) on the step-start lists */
+            int st = dg_set_forces(c);
+            if (st) return st;
+            dg_average_pass(c, 0);
+            dg_average_pass(c, 1);
+        }
         double mx_e = 0.0, mx_m = 0.0;
         int64_t warn = 0;
         int err = 0;

@@ -52,6 +52,11 @@ typedef struct orc_params {
     const double* dg_value;         /* nphases*nphases, antisymmetric (UDF_Value_n_m) */
     double dg_limit;                /* DrivingForce::Limit (0.95 default) */
     double regularization_factor;   /* InterfaceProperties::RegularizationFactor (1.0) */
+    /* DrivingForce::Average (src/DrivingForce.cpp:242-255, 293-478): 0 = SkipAverage */
+    int32_t dg_average;
+    int32_t dg_range;               /* Range in cells; <= 0: (int)iwidth (DrivingForce.cpp:64) */
+    double dg_threshold;            /* PhiThreshold; <= 0: iwidth < 5 ? iwidth/15 : 1/3 (:66-73) */
+    int32_t dg_weights;             /* WeightsMode: 0 PHASEFIELDS, 1 RANGE, 2 COUNTER (:135-152) */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

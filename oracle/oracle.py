@@ -28,6 +28,8 @@ class OrcParams(C.Structure):
         ("consider_nucleus_volume", C.c_int32), ("nucleus_volume_factor", C.c_double),
         ("dg_mode", C.POINTER(C.c_int32)), ("dg_value", C.POINTER(C.c_double)),
         ("dg_limit", C.c_double), ("regularization_factor", C.c_double),
+        ("dg_average", C.c_int32), ("dg_range", C.c_int32), ("dg_threshold", C.c_double),
+        ("dg_weights", C.c_int32),
     ]
 
 
@@ -99,6 +101,10 @@ class Oracle:
         p.dg_value = _p(arr(cfg.dg_value, np.float64), C.c_double)
         p.dg_limit = cfg.dg_limit
         p.regularization_factor = cfg.regularization_factor
+        p.dg_average = int(cfg.dg_average)
+        p.dg_range = int(cfg.dg_range)
+        p.dg_threshold = float(cfg.dg_threshold)
+        p.dg_weights = int(cfg.dg_weights)
         self._p = p
         self.h = None
         self.ngrains = 0
@@ -214,7 +220,13 @@ def write_opi(path, cfg, threads: int = 1) -> None:
             for b in range(a + 1, cfg.nphases):
                 if mode[a, b] == 1:
                     lines += [f"$UDF_Mode_{a}_{b} : VALUE", f"$UDF_Value_{a}_{b} : {float(val[a, b])!r}"]
-        lines += ["", "@DrivingForce", "$Average : No"]
+        lines += ["", "@DrivingForce", f"$Average : {'Yes' if cfg.dg_average else 'No'}"]
+        if cfg.dg_average:
+            if cfg.dg_range > 0:
+                lines.append(f"$Range : {int(cfg.dg_range)}")
+            if cfg.dg_threshold > 0.0:
+                lines.append(f"$Threshold : {float(cfg.dg_threshold)!r}")
+            lines.append(f"$WeightsMode : {('PHASEFIELDS', 'RANGE', 'COUNTER')[cfg.dg_weights]}")
         for a in range(cfg.nphases):
             for b in range(a, cfg.nphases):
                 lines.append(f"$Limit_{a}_{b} : {cfg.dg_limit!r}")

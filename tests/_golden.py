@@ -25,6 +25,10 @@ class Case:
             tj_model=int(z["tj_model"]), tj_factor=float(z["tj_factor"]), tj_energy=float(z["tj_energy"]),
             dg_mode=z["dg_mode"] if bool(z["has_dg"]) else None,
             dg_value=z["dg_value"] if bool(z["has_dg"]) else None,
+            dg_average=bool(z["dg_average"]) if "dg_average" in z else False,
+            dg_range=int(z["dg_range"]) if "dg_range" in z else 0,
+            dg_threshold=float(z["dg_threshold"]) if "dg_threshold" in z else 0.0,
+            dg_weights=int(z["dg_weights"]) if "dg_weights" in z else 0,
             precision=precision,
             slots=16,  # the reference's lists are unbounded; the densest fixture needs 9
         )

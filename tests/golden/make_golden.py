@@ -5,7 +5,7 @@ seeded microstructures and stores, per case, the input state, the parameters and
 state (lists, grain volumes, volume ratios) at a few steps. The tests compare the CPU oracle and
 the GPU path against these files; the reference itself is never needed at test time.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
 
@@ -52,11 +52,31 @@ def cases():
                        dg_mode=np.array([[0, 1], [1, 0]], np.int32),
                        dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]))
     out.append(("aniso_dg2d_32", aniso, 12, 16, phase2, q, 30, [0, 1, 10, 30], 0.9))
+    # driving force averaged over the interface (DrivingForce Average: Yes), three phases with
+    # different constant driving forces, default Range / Threshold / PHASEFIELDS weights
+    dg3 = np.array([[0.0, 1e5, -5e4], [-1e5, 0.0, 2e5], [5e4, -2e5, 0.0]])
+    three = StepConfig(32, 32, 0, nphases=3, sigma=np.full((3, 3), 0.24), mu=np.full((3, 3), 4e-9),
+                       dg_mode=np.ones((3, 3), np.int32) - np.eye(3, dtype=np.int32), dg_value=dg3,
+                       dg_average=True)
+    phase3 = (np.arange(12) % 3).astype(np.int32)
+    out.append(("avg_dg2d_32", three, 12, 17, phase3, None, 30, [0, 1, 10, 30], 0.9))
+    # RANGE weights, Range 4, Threshold 0.2, 3D
+    r3 = StepConfig(16, 16, 16, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                    dg_mode=np.array([[0, 1], [1, 0]], np.int32), dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]),
+                    dg_average=True, dg_range=4, dg_threshold=0.2, dg_weights=1)
+    out.append(("avg_range3d_16", r3, 10, 18, (np.arange(10) % 2).astype(np.int32), None, 12, [0, 1, 6, 12], 0.9))
+    # COUNTER weights with grains below the reference volume (VolumeRatio < 1 scales the threshold)
+    cnt = StepConfig(24, 24, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                     dg_mode=np.array([[0, 1], [1, 0]], np.int32), dg_value=np.array([[0.0, 2e5], [-2e5, 0.0]]),
+                     dg_average=True, dg_weights=2)
+    out.append(("avg_counter2d_24", cnt, 30, 19, (np.arange(30) % 2).astype(np.int32), None, 30, [0, 1, 5, 30], 1.0))
     return out
 
 
-def main():
+def main(only=()):
     for name, cfg, ng, seed, phase, quat, nsteps, dumps, dtf in cases():
+        if only and name not in only:
+            continue
         lists = voronoi_grains(cfg.nx, cfg.ny, cfg.nz, ng, seed=seed)
         phase = np.zeros(ng, np.int32) if phase is None else phase
         dt = cfg.max_time_step() * dtf
@@ -70,6 +90,8 @@ def main():
             epsilon=np.zeros((cfg.nphases, cfg.nphases)) if cfg.epsilon is None else np.asarray(cfg.epsilon),
             tj_model=cfg.tj_model, tj_factor=cfg.tj_factor, tj_energy=cfg.tj_energy,
             has_dg=cfg.dg_mode is not None,
+            dg_average=int(cfg.dg_average), dg_range=int(cfg.dg_range), dg_threshold=float(cfg.dg_threshold),
+            dg_weights=int(cfg.dg_weights),
             dg_mode=np.zeros((cfg.nphases, cfg.nphases), np.int32) if cfg.dg_mode is None else cfg.dg_mode,
             dg_value=np.zeros((cfg.nphases, cfg.nphases)) if cfg.dg_value is None else cfg.dg_value,
             dt=dt, grain_phase=phase, quaternions=np.zeros((ng, 4)) if quat is None else quat,
@@ -88,4 +110,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _domains():
-    """Three 2D domains of different sizes, grain counts and tables (one in FP32)."""
+    """Three 2D grain-growth domains of different sizes, grain counts and tables (one in FP32)."""
     specs = [
         (48, 48, 9, 0.24, 4e-9, 0, 11),
         (64, 40, 14, 0.35, 2e-9, 0, 12),
@@ -24,9 +24,17 @@ def _domains():
     return out
 
 
+def _dg_domain():
+    """Two phases, constant driving force averaged over the interface."""
+    cfg = StepConfig(40, 52, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                     dg_mode=np.array([[0, 1], [1, 0]], np.int32), dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]),
+                     dg_average=True, slots=8)
+    return cfg, voronoi_grains(40, 52, 0, 10, seed=14), 10, 0.9 * cfg.max_time_step()
+
+
 def _make(cfg, lists, ng):
     e = Engine(cfg)
-    e.set_grains(np.zeros(ng, np.int32))
+    e.set_grains((np.arange(ng) % cfg.nphases).astype(np.int32))
     e.upload(lists)
     e.finalize_initialization()
     return e
@@ -41,7 +49,7 @@ def _same(a, b):
 
 
 def test_ensemble_matches_members_stepped_alone():
-    doms = _domains()
+    doms = _domains() + [_dg_domain()]
     solo = [_make(c, l, g) for c, l, g, _ in doms]
     team = [_make(c, l, g) for c, l, g, _ in doms]
     dts = [d for *_, d in doms]

@@ -232,3 +232,44 @@ def test_one_step_matches_python_brute_force():
         assert [g for g, _ in mine] == [g for g, _ in exp[c]], c
         for (_, a), (_, b) in zip(mine, exp[c]):
             assert a == pytest.approx(b, abs=1e-15), c
+
+
+@pytest.mark.parametrize("name", ["avg_dg2d_32", "avg_counter2d_24"])
+def test_golden_pins_driving_force_averaging(name):
+    """The averaging fixtures separate DrivingForce::Average from SkipAverage: the oracle without
+    averaging misses them (so a dropped or broken averaging pass cannot pass the golden test)."""
+    import dataclasses
+
+    from _golden import Case
+
+    case = Case(name)
+    orc = case.setup(O.Oracle(dataclasses.replace(case.cfg, dg_average=False)))
+    differs = False
+    for s, got, _, _ in case.run(orc):
+        exp = case.expect[s][0]
+        differs = differs or not (np.array_equal(got.count, exp.count) and np.array_equal(got.index, exp.index)
+                                  and np.array_equal(got.value, exp.value))
+    orc.close()
+    assert differs
+
+
+def test_averaged_constant_driving_force_is_the_constant():
+    """Closed form: averaging a constant driving force returns the constant up to rounding wherever
+    weights are positive. With the threshold near zero every pair takes part in both passes, so one
+    step equals the unaveraged step to rounding."""
+    import dataclasses
+
+    from _golden import Case
+
+    case = Case("avg_dg2d_32")
+    b = case.setup(O.Oracle(dataclasses.replace(case.cfg, dg_threshold=1e-9)))
+    c = case.setup(O.Oracle(dataclasses.replace(case.cfg, dg_average=False)))
+    for o in (b, c):
+        o.step(case.dt, 1)
+    lb, lc = b.download(), c.download()
+    # a threshold near zero lets every pair average itself: the result equals SkipAverage to rounding
+    # (a sign slip in the pair orientation or a wrong normalisation would move it by O(1))
+    assert np.array_equal(lb.count, lc.count) and np.array_equal(lb.index, lc.index)
+    assert np.abs(lb.value - lc.value).max() < 1e-13
+    for o in (b, c):
+        o.close()
