@@ -232,6 +232,43 @@ def run_ensemble(steps: int):
     }
 
 
+def run_dg_average(steps: int):
+    """Driving force averaged over the interface (DrivingForce::Average, PHASEFIELDS weights,
+    Range 5, Threshold 1/3) on the calibration microstructure: 2D 512^2 periodic, 200 grains of two
+    phases, isotropic sigma 0.24 / mu 4e-9, constant driving force 1e5 J/m^3, dt = 0.9 x the 2D
+    limit. Cell-updates/s with and without averaging, FP64 and FP32 (wall time of `steps` steps)."""
+    from openphase_amd.engine import Engine, StepConfig
+    from openphase_amd.microstructure import voronoi_grains
+
+    n, ng = 512, 200
+    rng = np.random.default_rng(0x0FE4)
+    phase = np.zeros(ng, np.int32)
+    phase[rng.permutation(ng)[: ng // 2]] = 1
+    lists = voronoi_grains(n, n, 0, ng, seed=0x0FE4)
+    out = {"steps": steps,
+           "workload": "2D 512^2 periodic, 200 grains, 2 phases, driving force 1e5 J/m^3, averaged over "
+                       "the interface (Range 5, PHASEFIELDS weights) vs not averaged"}
+    for prec, tag in ((0, "fp64"), (1, "fp32")):
+        for avg in (False, True):
+            cfg = StepConfig(n, n, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                             dg_mode=np.array([[0, 1], [1, 0]], np.int32),
+                             dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]), dg_average=avg, precision=prec, slots=8)
+            dt = 0.9 * cfg.max_time_step()
+            e = Engine(cfg)
+            e.set_grains(phase)
+            e.upload(lists)
+            e.finalize_initialization()
+            e.step(dt, 5)
+            _device_sync()
+            t0 = time.perf_counter()
+            e.step(dt, steps)
+            _device_sync()
+            t = time.perf_counter() - t0
+            e.close()
+            out[f"{tag}_{'averaged' if avg else 'plain'}_cell_updates_per_s"] = n * n * steps / t
+    return out
+
+
 def cpu_baseline(lists, budget_s: float = 30.0):
     """Times the reference OpenPhase build (oracle/_ref/ref_driver, compiled from the reference
     sources) on this host: same 256^3 state, one step, all available cores (max 16)."""
@@ -267,6 +304,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--adjoint-steps", type=int, default=100, help="rollout length of the adjoint leg")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the ensemble leg")
+    ap.add_argument("--dg-steps", type=int, default=200, help="steps of the driving-force averaging leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the step on a single MI355X "
@@ -286,6 +324,8 @@ def main():
     print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
     ens = run_ensemble(args.ensemble_steps)
     print(f"[bench] ensemble {ens}", file=sys.stderr, flush=True)
+    dga = run_dg_average(args.dg_steps)
+    print(f"[bench] driving-force averaging {dga}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(lists)
     print(f"[bench] cpu {cpu}", file=sys.stderr, flush=True)
 
@@ -335,6 +375,7 @@ def main():
         },
         "adjoint": adj,
         "ensemble": ens,
+        "dg_average": dga,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -60,10 +60,14 @@ struct mpf_ctx {
     double prefactor = 0.0, dg_prefactor = 0.0;
     // driving-force averaging (DrivingForce::Average); tmp and averaged value per pair of own entries
     bool dg_avg = false;
-    int avg_range = 0, avg_weights = 0, avg_xr = 0, avg_yr = 0, avg_zr = 0;
+    int avg_range = 0, avg_weights = 0;
     double avg_threshold = 0.0;
+    std::vector<int32_t> avg_off;    // sphere offsets (3 each) and their distance weights
+    std::vector<double> avg_off_w;
     double* d_avg_tmp = nullptr;
     double* d_avg = nullptr;
+    int32_t* d_avg_off = nullptr;
+    double* d_avg_off_w = nullptr;
 
     hipStream_t stream = nullptr;
     uint8_t* cnt[2] = {nullptr, nullptr};
@@ -264,8 +268,9 @@ mpf::AvgArgs avg_args(const mpf_ctx* c, const mpf::StepArgs& s)
     a.s = s;
     a.tmp = c->d_avg_tmp;
     a.avg = c->d_avg;
-    a.range = c->avg_range;
-    a.xr = c->avg_xr; a.yr = c->avg_yr; a.zr = c->avg_zr;
+    a.off = c->d_avg_off;
+    a.off_w = c->d_avg_off_w;
+    a.noff = static_cast<int32_t>(c->avg_off_w.size());
     a.weights = c->avg_weights;
     a.threshold = c->avg_threshold;
     return a;
@@ -419,9 +424,18 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         // the reference's periodic halo of Range cells is a wrapped index only on axes of >= Range cells
         if ((c->dnx && c->nx < c->avg_range) || (c->dny && c->ny < c->avg_range) || (c->dnz && c->nz < c->avg_range))
             return bail(MPF_ERR_UNSUPPORTED, "driving-force averaging needs every active axis >= Range cells");
-        c->avg_xr = std::min(c->avg_range, c->nx - 1) * c->dnx;
-        c->avg_yr = std::min(c->avg_range, c->ny - 1) * c->dny;
-        c->avg_zr = std::min(c->avg_range, c->nz - 1) * c->dnz;
+        // CollectAverage / DistributeAverage loop bounds and distance weights (src/DrivingForce.cpp:295-324)
+        const int xr = std::min(c->avg_range, c->nx - 1) * c->dnx;
+        const int yr = std::min(c->avg_range, c->ny - 1) * c->dny;
+        const int zr = std::min(c->avg_range, c->nz - 1) * c->dnz;
+        for (int ii = -xr; ii <= xr; ii++)
+            for (int jj = -yr; jj <= yr; jj++)
+                for (int kk = -zr; kk <= zr; kk++) {
+                    const double wd = c->avg_range - std::sqrt(static_cast<double>(ii * ii + jj * jj + kk * kk));
+                    if (!(wd > 0.0)) continue;
+                    c->avg_off.insert(c->avg_off.end(), {ii, jj, kk});
+                    c->avg_off_w.push_back(wd);
+                }
     }
 
     // Stencil weights by number of non-zero offset components: Laplacian / dx^2, gradient / dx
@@ -462,7 +476,9 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         return bail(MPF_ERR_NOMEM, c->err.c_str());
     if (c->dg_avg) {
         const size_t pairs = static_cast<size_t>(c->slots) * (c->slots - 1) / 2;
-        if (c->hip(dalloc(&c->d_avg_tmp, pairs * N), "averaging arrays") || c->hip(dalloc(&c->d_avg, pairs * N), "averaging arrays"))
+        if (c->hip(dalloc(&c->d_avg_tmp, pairs * N), "averaging arrays") || c->hip(dalloc(&c->d_avg, pairs * N), "averaging arrays") ||
+            c->hip(dalloc(&c->d_avg_off, c->avg_off.size()), "averaging arrays") ||
+            c->hip(dalloc(&c->d_avg_off_w, c->avg_off_w.size()), "averaging arrays"))
             return bail(MPF_ERR_NOMEM, c->err.c_str());
     }
     hipError_t e = hipSuccess;
@@ -474,6 +490,10 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     h2d(e, c->d_dgmode, c->dgmode.data(), np2);
     h2d(e, c->d_dgval, c->dgval.data(), np2);
     h2d(e, c->d_dgallowed, c->dgallowed.data(), np2);
+    if (c->dg_avg) {
+        h2d(e, c->d_avg_off, c->avg_off.data(), c->avg_off.size());
+        h2d(e, c->d_avg_off_w, c->avg_off_w.data(), c->avg_off_w.size());
+    }
     const int32_t zeros[mpf::kStatusWords] = {0, 0, 0, 0};
     const unsigned long long none = ~0ull, zero = 0ull;
     h2d(e, c->d_status, zeros, mpf::kStatusWords);
@@ -855,7 +875,7 @@ void mpf_destroy(mpf_ctx* c)
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
     dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats);
-    dfree(c->d_avg_tmp); dfree(c->d_avg);
+    dfree(c->d_avg_tmp); dfree(c->d_avg); dfree(c->d_avg_off); dfree(c->d_avg_off_w);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

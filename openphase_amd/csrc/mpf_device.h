@@ -110,8 +110,12 @@ struct AvgArgs {
     StepArgs s;                  // state of the step start (cnt_in, id_in, val_in), grains, tables
     double* tmp;
     double* avg;
-    int32_t range;               // Range (sphere radius, cells)
-    int32_t xr, yr, zr;          // offset bound per axis: min(Range, N - 1) on active axes, else 0
+    // offsets of the averaging sphere in the reference's loop order (x outer, z inner; each axis
+    // within min(Range, N - 1), 0 on reduced axes) that have Range - |offset| > 0: 3 ints each,
+    // and that distance weight
+    const int32_t* off;
+    const double* off_w;
+    int32_t noff;
     int32_t weights;             // 0 PHASEFIELDS, 1 RANGE, 2 COUNTER
     double threshold;            // sqrt(PhiThreshold * (1 - PhiThreshold))
 };

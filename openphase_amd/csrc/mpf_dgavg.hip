@@ -7,10 +7,10 @@
 // raw by its weighted mean over the sphere of radius Range (tmp); DistributeAverage (:381-478)
 // averages tmp again (average), which MergePhaseFieldIncrementsSR then limits and merges. Here no
 // Force storage exists: an entry's raw value and weight follow from the node's own list (grains,
-// values) and the phase-pair table, so both passes read the step-start state directly. One thread
-// per cell walks its own pairs and, per pair, the offsets x outer, z inner (the reference's loop
-// order, so every sum is formed in the same order); a neighbour contributes when it is an
-// interface cell holding both grains. tmp and the averaged value are stored per pair of own
+// values) and the phase-pair table, so both passes read the step-start state directly. A
+// wavefront takes one interface cell at a time and, per own pair, the sphere's offsets x outer,
+// z inner (the reference's loop order; every sum is formed in that order); a neighbour contributes
+// when it is an interface cell holding both grains. tmp and the averaged value are stored per pair of own
 // entries (a < b at plane b(b-1)/2 + a) for the step kernel, which reads them in place of the
 // phase-pair value.
 //
@@ -72,76 +72,115 @@ __device__ bool find_pair(const StepArgs& A, int64_t nb, uint32_t ga, uint32_t g
     return true;
 }
 
-// SECOND = false: CollectAverage into tmp; true: DistributeAverage into avg.
+// Lane l's value of x (l uniform across the wavefront).
+__device__ __forceinline__ double lane_value(double x, int l)
+{
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b)), l));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b >> 32)), l));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+constexpr int kAvgCells = 64;                 // cells per block
+constexpr int kAvgWaves = kAvgBlock / 64;     // wavefronts per block
+
+// SECOND = false: CollectAverage into tmp; true: DistributeAverage into avg. A block takes 64
+// consecutive cells and compacts their interface cells in LDS; its wavefronts take them in turn.
+// For one cell and pair the 64 lanes evaluate 64 sphere offsets at once (the neighbour's weight
+// and raw or tmp value), then every lane adds the terms of the lanes that count in lane order,
+// chunk after chunk: the sums are formed in the reference's offset order, and a wavefront waits
+// on a few memory round trips per chunk instead of one chain per offset.
 template <typename TV, bool SECOND>
 __global__ __launch_bounds__(kAvgBlock) void dg_average_kernel(const AvgArgs P)
 {
+    __shared__ uint16_t s_list[kAvgCells];
+    __shared__ int s_count;
     const StepArgs& A = P.s;
     const int64_t N = A.ncells;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kAvgBlock + threadIdx.x;
-    if (c >= N) return;
-    const uint8_t code = A.cnt_in[c];
-    const int n = ListCode<TV>::count(code);
-    if (n < 2) return;  // Force entries live on interface cells only
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kAvgCells;
+    if (wave == 0) {
+        // Force entries live on interface cells (more than one own entry) only
+        const int64_t c0 = base + lane;
+        const bool inter = lane < kAvgCells && c0 < N && ListCode<TV>::count(A.cnt_in[c0]) >= 2;
+        const unsigned long long m = __ballot(inter);
+        if (inter) s_list[__popcll(m & ((1ull << lane) - 1ull))] = static_cast<uint16_t>(lane);
+        if (lane == 0) s_count = __popcll(m);
+    }
+    __syncthreads();
+    const int ninter = s_count;
     const int64_t nyz = static_cast<int64_t>(A.ny) * A.nz;
-    const int x = static_cast<int>(c / nyz), y = static_cast<int>((c / A.nz) % A.ny), z = static_cast<int>(c % A.nz);
     const int NP = A.nphases;
-    const double R = static_cast<double>(P.range);
-    for (int b = 1; b < n; ++b)
-        for (int a = 0; a < b; ++a) {
-            const uint32_t ga = A.id_in[static_cast<int64_t>(a) * N + c], gb = A.id_in[static_cast<int64_t>(b) * N + c];
-            const int pa = A.gphase[ga], pb = A.gphase[gb];
-            if (A.dg_mode[pa * NP + pb] != 1) continue;  // no Force entry for this pair
-            const int plane = pair_plane(a, b);
-            const double w = sqrt(entry_value<TV>(A, c, code, a) * entry_value<TV>(A, c, code, b));
-            const double scale = A.gvr[ga] * A.gvr[gb];
-            const double cut = P.threshold * scale;
-            const double vab = A.dg_value[pa * NP + pb], vba = A.dg_value[pb * NP + pa];
-            // own value the pass falls back to: raw (collect) or tmp (distribute)
-            const double own = SECOND ? P.tmp[static_cast<int64_t>(plane) * N + c] : vab;
-            double result = 0.0;  // a fresh entry's tmp / average
-            if (SECOND ? (w > 0.0) : (w > cut)) {
-                double value = 0.0, sum = 0.0;
-                int counter = 0;
-                for (int ii = -P.xr; ii <= P.xr; ii++)
-                    for (int jj = -P.yr; jj <= P.yr; jj++)
-                        for (int kk = -P.zr; kk <= P.zr; kk++) {
-                            const double wdist = R - sqrt(static_cast<double>(ii * ii + jj * jj + kk * kk));
-                            if (!(wdist > 0.0)) continue;
+    for (int i = wave; i < ninter; i += kAvgWaves) {
+        const int64_t c = base + s_list[i];
+        const uint8_t code = A.cnt_in[c];
+        const int n = ListCode<TV>::count(code);
+        const int x = static_cast<int>(c / nyz), y = static_cast<int>((c / A.nz) % A.ny), z = static_cast<int>(c % A.nz);
+        for (int b = 1; b < n; ++b)
+            for (int a = 0; a < b; ++a) {
+                const uint32_t ga = A.id_in[static_cast<int64_t>(a) * N + c], gb = A.id_in[static_cast<int64_t>(b) * N + c];
+                const int pa = A.gphase[ga], pb = A.gphase[gb];
+                if (A.dg_mode[pa * NP + pb] != 1) continue;  // no Force entry for this pair
+                const int plane = pair_plane(a, b);
+                const double w = sqrt(entry_value<TV>(A, c, code, a) * entry_value<TV>(A, c, code, b));
+                const double scale = A.gvr[ga] * A.gvr[gb];
+                const double cut = P.threshold * scale;
+                const double vab = A.dg_value[pa * NP + pb], vba = A.dg_value[pb * NP + pa];
+                // own value the pass falls back to: raw (collect) or tmp (distribute)
+                const double own = SECOND ? P.tmp[static_cast<int64_t>(plane) * N + c] : vab;
+                double result = 0.0;  // a fresh entry's tmp / average
+                if (SECOND ? (w > 0.0) : (w > cut)) {
+                    double value = 0.0, sum = 0.0;
+                    int counter = 0;
+                    for (int t0 = 0; t0 < P.noff; t0 += 64) {
+                        const int t = t0 + lane;
+                        bool ok = false;
+                        double wt = 0.0, term = 0.0;
+                        if (t < P.noff) {
+                            const int ii = P.off[3 * t], jj = P.off[3 * t + 1], kk = P.off[3 * t + 2];
                             const int64_t nb = (static_cast<int64_t>(wrap1(x + ii, A.nx)) * A.ny + wrap1(y + jj, A.ny)) * A.nz +
                                                wrap1(z + kk, A.nz);
                             double wphi = 0.0;
                             int np = 0;
                             bool fwd = true;
-                            if (!find_pair<TV>(A, nb, ga, gb, wphi, np, fwd)) continue;
-                            if (!(SECOND ? (wphi > cut) : (wphi > 0.0))) continue;
-                            double wt = 0.0;
-                            if (P.weights == 1) { wt = wdist; sum += wt; }
-                            else if (P.weights == 0) { wt = wphi; sum += wt; }
-                            else { counter++; wt = 1.0; }
-                            double v;
-                            if (SECOND) {
-                                const double t = P.tmp[static_cast<int64_t>(np) * N + nb];
-                                v = fwd ? t : -t;
-                            } else {
-                                v = fwd ? vab : -vba;
+                            if (find_pair<TV>(A, nb, ga, gb, wphi, np, fwd) && (SECOND ? (wphi > cut) : (wphi > 0.0))) {
+                                ok = true;
+                                wt = P.weights == 1 ? P.off_w[t] : (P.weights == 0 ? wphi : 1.0);
+                                double v;
+                                if (SECOND) {
+                                    const double tn = P.tmp[static_cast<int64_t>(np) * N + nb];
+                                    v = fwd ? tn : -tn;
+                                } else {
+                                    v = fwd ? vab : -vba;
+                                }
+                                term = wt * v;
                             }
-                            value += wt * v;
                         }
-                if (P.weights == 2) result = counter != 0 ? value / static_cast<double>(counter) : own;
-                else result = sum > 0.0 ? value / sum : own;
+                        // value += weight * v and sum += weight in offset order
+                        unsigned long long okm = __ballot(ok);
+                        while (okm) {
+                            const int l = __ffsll(static_cast<long long>(okm)) - 1;
+                            okm &= okm - 1ull;
+                            if (P.weights == 2) counter++;
+                            else sum += lane_value(wt, l);
+                            value += lane_value(term, l);
+                        }
+                    }
+                    if (P.weights == 2) result = counter != 0 ? value / static_cast<double>(counter) : own;
+                    else result = sum > 0.0 ? value / sum : own;
+                }
+                // DistributeAverage's weight-0 branch (Seed grains) cannot occur: own entries are
+                // above the finalize cutoff and no grain is a Seed
+                if (lane == 0) (SECOND ? P.avg : P.tmp)[static_cast<int64_t>(plane) * N + c] = result;
             }
-            // DistributeAverage's weight-0 branch (Seed grains) cannot occur: own entries are
-            // above the finalize cutoff and no grain is a Seed
-            (SECOND ? P.avg : P.tmp)[static_cast<int64_t>(plane) * N + c] = result;
-        }
+    }
 }
 
 }  // namespace
 
 int launch_dg_average(const AvgArgs& a, bool fp32, void* stream)
 {
-    const dim3 grid(static_cast<unsigned>((a.s.ncells + kAvgBlock - 1) / kAvgBlock));
+    const dim3 grid(static_cast<unsigned>((a.s.ncells + kAvgCells - 1) / kAvgCells));
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (fp32) {
         hipLaunchKernelGGL((dg_average_kernel<float, false>), grid, dim3(kAvgBlock), 0, s, a);
