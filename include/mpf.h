@@ -144,6 +144,11 @@ mpf_status mpf_download(mpf_ctx* ctx, int32_t* count, int32_t* index, double* va
 /* GrainsProperties Volume and VolumeRatio per grain (either may be NULL). */
 mpf_status mpf_get_grains(mpf_ctx* ctx, double* volume, double* volume_ratio);
 
+/* GrainsProperties MAXVolume, RefVolume, Exist and Stage (0 Seed, 1 Nucleus, 2 Stable) per grain,
+ * any may be NULL: the Grain fields GrainsProperties::Write stores (include/GrainsProperties.h:196-230)
+ * besides Phase and Orientation, which the caller set. */
+mpf_status mpf_get_grain_records(mpf_ctx* ctx, double* maxvolume, double* refvolume, uint8_t* exist, int32_t* stage);
+
 /* InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556) evaluated from
  * the phase-pair maxima of energy/mobility (ISO models). */
 mpf_status mpf_max_time_step(mpf_ctx* ctx, double* dt_max);

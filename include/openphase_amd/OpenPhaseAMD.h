@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "../mpf.h"
+#include "io.h"
 
 namespace openphase_amd {
 
@@ -140,9 +141,14 @@ class Settings {
         Solid.assign(Nphases, 1);
         for (size_t n = 0; n < Nphases; ++n)
             Solid[n] = in.get("Settings", "State_" + std::to_string(n), "SOLID") == "SOLID" ? 1 : 0;
+        // Settings::ReadInput (src/Settings.cpp:245-247): output directories
+        VTKDir = in.get("Settings", "VTKDir", "VTK/");
+        RawDataDir = in.get("Settings", "RAWDir", "RawData/");
         InputFileName = file;
     }
     GridParameters Grid;
+    std::string VTKDir = "VTK/";
+    std::string RawDataDir = "RawData/";
     size_t Nphases = 0;
     std::vector<std::string> PhaseNames;
     std::vector<uint8_t> Solid;
@@ -194,11 +200,13 @@ class PhaseField {
     PhaseField(const PhaseField&) = delete;
     PhaseField& operator=(const PhaseField&) = delete;
 
-    // PhaseField::AddGrainInfo (src/PhaseField.cpp:2970-2979)
-    size_t AddGrainInfo(size_t PhaseIndex, const double* rotation9 = nullptr)
+    // PhaseField::AddGrainInfo (src/PhaseField.cpp:2970-2979); `quaternion` (s, x, y, z) is the
+    // Orientation the grain table file stores (rotation9 is what the step uses)
+    size_t AddGrainInfo(size_t PhaseIndex, const double* rotation9 = nullptr, const double* quaternion = nullptr)
     {
         grain_phase_.push_back(static_cast<int32_t>(PhaseIndex));
         for (int q = 0; q < 9; ++q) grain_rot_.push_back(rotation9 ? rotation9[q] : (q % 4 == 0 ? 1.0 : 0.0));
+        for (int q = 0; q < 4; ++q) grain_quat_.push_back(quaternion ? quaternion[q] : 0.0);
         grains_dirty_ = true;
         return grain_phase_.size() - 1;
     }
@@ -239,38 +247,78 @@ class PhaseField {
         ensure_context();
         check(mpf_upload(ctx_, count, index, value), ctx_, "PhaseField::Read");
     }
-    // PhaseField::Write(FileName) (src/PhaseField.cpp:2340-2388)
+    // PhaseField::Write(FileName) (src/PhaseField.cpp:2340-2388): the lists as the reference holds
+    // them after a step, zero-valued appended entries included
     bool Write(const std::string& FileName) const
     {
         if (!ctx_) return false;
-        std::vector<int32_t> count, index;
-        std::vector<double> value;
-        Download(count, index, value);
-        std::ofstream out(FileName, std::ios::binary);
-        if (!out) return false;
-        int n[3] = {Grid.Nx ? Grid.Nx : 1, Grid.Ny ? Grid.Ny : 1, Grid.Nz ? Grid.Nz : 1};
-        out.write(reinterpret_cast<const char*>(n), sizeof(n));
-        size_t off = 0;
-        for (size_t c = 0; c < count.size(); ++c) {
-            uint64_t k = static_cast<uint64_t>(count[c]);
-            out.write(reinterpret_cast<const char*>(&k), sizeof(k));
-            for (uint64_t q = 0; q < k; ++q, ++off) {
-                uint64_t idx = static_cast<uint64_t>(index[off]);
-                out.write(reinterpret_cast<const char*>(&idx), sizeof(idx));
-                out.write(reinterpret_cast<const char*>(&value[off]), sizeof(double));
-            }
-        }
-        return static_cast<bool>(out);
+        io::Lists own;
+        std::vector<int8_t> flag;
+        Download(own.count, own.index, own.value, &flag);
+        return io::write_phasefield(FileName, dims(), io::full_lists(dims(), own, flag));
     }
-    void Download(std::vector<int32_t>& count, std::vector<int32_t>& index, std::vector<double>& value) const
+    // PhaseField::Write(Settings, tStep) (:2390-2401): PhaseField_<step>.dat and
+    // GrainsProperties_<step>.dat in RawDataDir
+    bool Write(const Settings& s, const int tStep) const
+    {
+        return Write(io::make_file_name(s.RawDataDir, "PhaseField_", tStep, ".dat")) &&
+               FieldsProperties.Write(io::make_file_name(s.RawDataDir, "GrainsProperties_", tStep, ".dat"));
+    }
+    // PhaseField::WriteVTK (src/PhaseField.cpp:2128-2158), single resolution: VTKDir/PhaseField_<step>.vts
+    void WriteVTK(Settings& s, const int tStep, const bool CurvatureOutput = false, const int precision = 16) const
+    {
+        if (CurvatureOutput) throw std::runtime_error("PhaseField::WriteVTK: curvature output is not implemented");
+        io::Lists own;
+        std::vector<int8_t> flag;
+        Download(own.count, own.index, own.value, &flag);
+        const std::string fn = io::make_file_name(s.VTKDir, "PhaseField_", tStep, ".vts");
+        if (!io::write_vtk(fn, dims(), static_cast<int>(Nphases), grain_records(), io::full_lists(dims(), own, flag),
+                           flag, precision))
+            throw std::runtime_error("PhaseField::WriteVTK: cannot write " + fn);
+    }
+    // GrainsProperties of the phase field: Write(FileName) (include/GrainsProperties.h:768-795)
+    struct GrainsTable {
+        const PhaseField* owner;
+        bool Write(const std::string& FileName) const { return io::write_grains(FileName, owner->grain_records()); }
+        size_t size() const { return owner->grain_phase_.size(); }
+    };
+    GrainsTable FieldsProperties{this};
+    // Grain records as GrainsProperties holds them (device volumes, stages; host phase, orientation)
+    std::vector<io::GrainRecord> grain_records() const
+    {
+        const size_t G = grain_phase_.size();
+        std::vector<io::GrainRecord> g(G);
+        if (!ctx_ || G == 0) return g;
+        std::vector<double> vol(G), mv(G), rv(G);
+        std::vector<uint8_t> ex(G);
+        std::vector<int32_t> st(G);
+        check(mpf_get_grains(ctx_, vol.data(), nullptr), ctx_, "GrainsProperties");
+        check(mpf_get_grain_records(ctx_, mv.data(), rv.data(), ex.data(), st.data()), ctx_, "GrainsProperties");
+        for (size_t i = 0; i < G; ++i) {
+            g[i].exist = ex[i] != 0;
+            g[i].stage = st[i];
+            g[i].phase = static_cast<size_t>(grain_phase_[i]);
+            g[i].state = static_cast<size_t>(grain_phase_[i]) < settings_.Solid.size() && !settings_.Solid[grain_phase_[i]] ? 1 : 0;
+            g[i].ref_volume = rv[i];
+            g[i].volume = vol[i];
+            g[i].max_volume = mv[i];
+            for (int q = 0; q < 4; ++q) g[i].orientation[q] = grain_quat_[4 * i + q];
+        }
+        return g;
+    }
+    void Download(std::vector<int32_t>& count, std::vector<int32_t>& index, std::vector<double>& value,
+                  std::vector<int8_t>* flag = nullptr) const
     {
         int64_t total = 0;
         check(mpf_total_entries(ctx_, &total), ctx_, "PhaseField::Download");
         count.resize(static_cast<size_t>(Grid.TotalNumberOfCells()));
         index.resize(static_cast<size_t>(total));
         value.resize(static_cast<size_t>(total));
-        check(mpf_download(ctx_, count.data(), index.data(), value.data(), nullptr), ctx_, "PhaseField::Download");
+        if (flag) flag->resize(count.size());
+        check(mpf_download(ctx_, count.data(), index.data(), value.data(), flag ? flag->data() : nullptr), ctx_,
+              "PhaseField::Download");
     }
+    io::Dims dims() const { return io::Dims::from_input(Grid.Nx, Grid.Ny, Grid.Nz); }
 
     // PhaseField::FinalizeInitialization (src/PhaseField.cpp:1068-1077)
     void FinalizeInitialization(const BoundaryConditions& BC)
@@ -335,6 +383,7 @@ class PhaseField {
     int32_t bc_[6] = {0, 0, 0, 0, 0, 0};
     std::vector<int32_t> grain_phase_;
     std::vector<double> grain_rot_;
+    std::vector<double> grain_quat_;
     bool grains_dirty_ = true;
     mpf_ctx* ctx_ = nullptr;
     int stage_ = 0;      // 0 idle, 1 IP.Set done, 2 increments done, 3 normalized

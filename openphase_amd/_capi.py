@@ -26,6 +26,7 @@ MPF_DG_WEIGHTS_PHASEFIELDS, MPF_DG_WEIGHTS_RANGE, MPF_DG_WEIGHTS_COUNTER = 0, 1,
 EXPORTED = [
     "mpf_create", "mpf_set_grains", "mpf_upload", "mpf_finalize_initialization", "mpf_step",
     "mpf_step_async", "mpf_synchronize", "mpf_total_entries", "mpf_download", "mpf_get_grains",
+    "mpf_get_grain_records",
     "mpf_max_time_step", "mpf_limiter_warnings", "mpf_overflow_cell", "mpf_device_state",
     "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
     "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_autotune",
@@ -89,6 +90,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_total_entries": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "mpf_download": ([vp, i32p, i32p, f64p, C.POINTER(C.c_int8)], C.c_int),
         "mpf_get_grains": ([vp, f64p, f64p], C.c_int),
+        "mpf_get_grain_records": ([vp, f64p, f64p, C.POINTER(C.c_uint8), C.POINTER(C.c_int32)], C.c_int),
         "mpf_max_time_step": ([vp, f64p], C.c_int),
         "mpf_limiter_warnings": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "mpf_overflow_cell": ([vp, C.POINTER(C.c_int64)], C.c_int),

@@ -756,6 +756,23 @@ mpf_status mpf_get_grains(mpf_ctx* c, double* volume, double* volume_ratio)
     return s;
 }
 
+mpf_status mpf_get_grain_records(mpf_ctx* c, double* maxvolume, double* refvolume, uint8_t* exist, int32_t* stage)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (!c->have_grains) return c->fail(MPF_ERR_STATE, "no grains");
+    const size_t G = static_cast<size_t>(c->ngrains);
+    mpf_status s = MPF_OK;
+    auto get = [&](void* dst, const void* src, size_t bytes) {
+        if (dst && !s) s = c->hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "grains");
+    };
+    get(maxvolume, c->g_maxvolume, G * sizeof(double));
+    get(refvolume, c->g_refvolume, G * sizeof(double));
+    get(exist, c->g_exist, G);
+    get(stage, c->g_stage, G * sizeof(int32_t));
+    if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    return s;
+}
+
 mpf_status mpf_max_time_step(mpf_ctx* c, double* dt_max)
 {
     if (!c || !dt_max) return MPF_ERR_ARG;

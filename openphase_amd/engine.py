@@ -185,6 +185,14 @@ class Engine:
         self._check(self.lib.mpf_get_grains(self.h, ptr(vol, C.c_double), ptr(vr, C.c_double)), "mpf_get_grains")
         return vol, vr
 
+    def grain_records(self):
+        """(MAXVolume, RefVolume, Exist, Stage) per grain."""
+        mv, rv = np.zeros(self.ngrains), np.zeros(self.ngrains)
+        ex, st = np.zeros(self.ngrains, np.uint8), np.zeros(self.ngrains, np.int32)
+        self._check(self.lib.mpf_get_grain_records(self.h, ptr(mv, C.c_double), ptr(rv, C.c_double),
+                                                   ptr(ex, C.c_uint8), ptr(st, C.c_int32)), "mpf_get_grain_records")
+        return mv, rv, ex, st
+
     def limiter_warnings(self) -> int:
         w = C.c_int64()
         self._check(self.lib.mpf_limiter_warnings(self.h, C.byref(w)), "mpf_limiter_warnings")

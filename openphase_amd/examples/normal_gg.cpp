@@ -2,7 +2,9 @@
 // the time loop of the reference's examples/NormalGG/NormalGG.cpp:74-85, unchanged apart from the
 // namespace, on a microstructure read from a raw PhaseField .dat file.
 //
-// usage: normal_gg <input.opi> <state.dat> <ngrains> <nsteps> <out.dat>
+// usage: normal_gg <input.opi> <state.dat> <ngrains> <nsteps> <out.dat> [outdir]
+// With outdir, also the reference's output files of the last step there: PhaseField::WriteVTK
+// (VTK/PhaseField_<step>.vts) and GrainsProperties::Write (GrainsProperties.dat).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -41,6 +43,11 @@ int main(int argc, char* argv[])
             Phi.MergeIncrements(BC, dt);
         }
         if (!Phi.Write(argv[5])) return 4;
+        if (argc > 6) {
+            OPSettings.VTKDir = std::string(argv[6]) + "/VTK/";
+            Phi.WriteVTK(OPSettings, nsteps);
+            if (!Phi.FieldsProperties.Write(std::string(argv[6]) + "/GrainsProperties.dat")) return 4;
+        }
         std::printf("normal_gg: %d steps at dt=%.17g\n", nsteps, dt);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "normal_gg: %s\n", e.what());

@@ -1002,6 +1002,17 @@ int orc_grains(const orc_ctx* c, double* volume, double* volume_ratio)
     return ORC_OK;
 }
 
+int orc_grain_records(const orc_ctx* c, double* maxvolume, double* refvolume, uint8_t* exist, int32_t* stage)
+{
+    for (int g = 0; g < c->ngrains; ++g) {
+        if (maxvolume) maxvolume[g] = c->maxvolume[g];
+        if (refvolume) refvolume[g] = c->refvolume[g];
+        if (exist) exist[g] = c->exist[g];
+        if (stage) stage[g] = c->stage[g];
+    }
+    return ORC_OK;
+}
+
 int64_t orc_limiter_warnings(const orc_ctx* c) { return c->warnings; }
 
 double orc_max_time_step(const orc_ctx* c)

@@ -80,6 +80,8 @@ int64_t orc_total_entries(const orc_ctx* c);
 int orc_download(const orc_ctx* c, int32_t* count, int32_t* index, double* value, int8_t* flag);
 /* Per-grain Volume and VolumeRatio. */
 int orc_grains(const orc_ctx* c, double* volume, double* volume_ratio);
+/* GrainsProperties MAXVolume, RefVolume, Exist and Stage per grain (any may be NULL). */
+int orc_grain_records(const orc_ctx* c, double* maxvolume, double* refvolume, uint8_t* exist, int32_t* stage);
 /* Limiter plausibility warnings seen so far (PhaseField.cpp:1536-1561). */
 int64_t orc_limiter_warnings(const orc_ctx* c);
 /* InterfaceProperties::ReportMaximumTimeStep for the last IP.Set (src/InterfaceProperties.cpp:523-556). */

@@ -53,6 +53,7 @@ def load():
         lib.orc_total_entries.restype = C.c_int64
         lib.orc_download.argtypes = [vp, i32p, i32p, f64p, C.POINTER(C.c_int8)]
         lib.orc_grains.argtypes = [vp, f64p, f64p]
+        lib.orc_grain_records.argtypes = [vp, f64p, f64p, C.POINTER(C.c_uint8), C.POINTER(C.c_int32)]
         lib.orc_limiter_warnings.argtypes = [vp]
         lib.orc_limiter_warnings.restype = C.c_int64
         lib.orc_max_time_step.argtypes = [vp]
@@ -161,6 +162,13 @@ class Oracle:
         self.lib.orc_grains(self.h, _p(vol, C.c_double), _p(vr, C.c_double))
         return vol, vr
 
+    def grain_records(self):
+        """(MAXVolume, RefVolume, Exist, Stage) per grain."""
+        mv, rv = np.zeros(self.ngrains), np.zeros(self.ngrains)
+        ex, st = np.zeros(self.ngrains, np.uint8), np.zeros(self.ngrains, np.int32)
+        self.lib.orc_grain_records(self.h, _p(mv, C.c_double), _p(rv, C.c_double), _p(ex, C.c_uint8), _p(st, C.c_int32))
+        return mv, rv, ex, st
+
     def limiter_warnings(self):
         return int(self.lib.orc_limiter_warnings(self.h))
 
@@ -233,9 +241,12 @@ def write_opi(path, cfg, threads: int = 1) -> None:
     Path(path).write_text("\n".join(lines) + "\n")
 
 
-def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dumps, threads=1, timeout=3600):
+def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dumps, threads=1, timeout=3600,
+                  write_files=False):
     """Runs the real reference (oracle/_ref/ref_driver) on `lists`; returns the parsed timing JSON
-    and {step: (Lists, volume, volume_ratio)} for the dump steps."""
+    and {step: (Lists, volume, volume_ratio)} for the dump steps. With write_files, each dump
+    step's entry also holds the bytes of the reference's own output files: {"dat": PhaseField::Write,
+    "grains": GrainsProperties::Write, "vtk": PhaseField::WriteVTK}."""
     import json
 
     from openphase_amd import rawio
@@ -250,6 +261,9 @@ def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dum
            repr(float(dt)), str(workdir / "out"), ",".join(str(s) for s in dumps), "--threads", str(threads)]
     if cfg.dg_mode is not None:
         cmd.append("--driving-force")
+    if write_files:
+        cmd.append("--write-files")
+        (workdir / "VTK").mkdir(exist_ok=True)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     res = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env, cwd=workdir)
     if res.returncode != 0:
@@ -261,4 +275,8 @@ def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dum
         st = rawio.read_phasefield(str(workdir / f"out_{s}.dat"))
         vol = np.fromfile(workdir / f"out_{s}.vol", dtype=np.float64)
         out[s] = (st, vol[:ng], vol[ng:2 * ng])
+        if write_files:
+            out[s] = out[s] + ({"dat": (workdir / f"out_{s}_full.dat").read_bytes(),
+                                "grains": (workdir / f"out_{s}.grains").read_bytes(),
+                                "vtk": (workdir / "VTK" / f"PhaseField_{s:08d}.vts").read_bytes()},)
     return timing, out

@@ -10,7 +10,10 @@
 // zero-valued entries that CalculateDerivativesSR appends (src/PhaseField.cpp:566-615).
 //
 // usage: ref_driver <input.opi> <state.dat> <grains.bin> <nsteps> <dt> <outprefix> <dumpsteps,...>
-//        [--driving-force] [--threads N]
+//        [--driving-force] [--threads N] [--write-files]
+// --write-files: at the dump steps also the reference's own writers: PhaseField::Write (full lists,
+// zero-valued entries included) to <outprefix>_<step>_full.dat, GrainsProperties::Write to
+// <outprefix>_<step>.grains and PhaseField::WriteVTK into the settings' VTK directory.
 // grains.bin: uint32 n; n x { uint32 phase; double q[4] }   (q = s, x, y, z)
 // Writes <outprefix>_<step>.dat and <outprefix>_<step>.vol (n doubles Volume, n doubles VolumeRatio)
 // and prints one JSON line with per-call wall times.
@@ -84,12 +87,13 @@ int main(int argc, char* argv[])
         std::string tok;
         while (std::getline(ss, tok, ',')) if (!tok.empty()) dumps.insert(std::atoi(tok.c_str()));
     }
-    bool with_dg = false;
+    bool with_dg = false, write_files = false;
     int threads = 1;
     for (int a = 8; a < argc; ++a)
     {
         std::string s = argv[a];
         if (s == "--driving-force") with_dg = true;
+        if (s == "--write-files") write_files = true;
         if (s == "--threads" && a + 1 < argc) threads = std::atoi(argv[++a]);
     }
     omp_set_num_threads(threads);
@@ -147,6 +151,12 @@ int main(int argc, char* argv[])
         double t5 = omp_get_wtime();
         t_ip += t1 - t0; t_do += t2 - t1; t_dg += t3 - t2; t_norm += t4 - t3; t_merge += t5 - t4;
         if (dumps.count(step)) dump_state(Phi, prefix, step);
+        if (dumps.count(step) && write_files)
+        {
+            Phi.Write(prefix + "_" + std::to_string(step) + "_full.dat");
+            Phi.FieldsProperties.Write(prefix + "_" + std::to_string(step) + ".grains");
+            Phi.WriteVTK(OPSettings, step);
+        }
     }
     long cells = (long)Phi.Grid.Nx * Phi.Grid.Ny * Phi.Grid.Nz;
     double total = t_ip + t_do + t_dg + t_norm + t_merge;

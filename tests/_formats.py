@@ -1,0 +1,86 @@
+"""Shared helpers of the output-format tests: run tests/io_driver.cpp (the io.h writers) on a
+finalized state and return the bytes of the three files it writes."""
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from openphase_amd import rawio
+
+ROOT = Path(__file__).resolve().parents[1]
+GRAIN_DTYPE = np.dtype([("exist", "u1"), ("stage", "<i4"), ("phase", "<i4"), ("state", "<i4"),
+                        ("refvolume", "<f8"), ("volume", "<f8"), ("maxvolume", "<f8"), ("q", "<f8", (4,))])
+
+
+def build_driver(tmp: Path) -> Path:
+    exe = tmp / "io_driver"
+    if not exe.exists():
+        subprocess.run(["g++", "-std=c++17", "-O2", "-I", str(ROOT / "include"), str(ROOT / "tests" / "io_driver.cpp"),
+                        "-o", str(exe)], check=True)
+    return exe
+
+
+def write_files(tmp: Path, cfg, lists, flags, phase, quaternions, records, volume) -> dict:
+    """records = (MAXVolume, RefVolume, Exist, Stage). Returns {"dat", "grains", "vtk"} bytes."""
+    exe = build_driver(tmp)
+    rawio.write_phasefield(str(tmp / "state.dat"), lists)
+    np.asarray(flags, np.int8).tofile(tmp / "flags.bin")
+    mv, rv, ex, st = records
+    n = len(phase)
+    g = np.zeros(n, GRAIN_DTYPE)
+    g["exist"], g["stage"], g["phase"], g["state"] = ex, st, phase, 0
+    g["refvolume"], g["volume"], g["maxvolume"] = rv, volume, mv
+    g["q"] = np.zeros((n, 4)) if quaternions is None else quaternions
+    with open(tmp / "grains.bin", "wb") as f:
+        f.write(np.uint32(n).tobytes())
+        f.write(g.tobytes())
+    subprocess.run([str(exe), str(cfg.nx), str(cfg.ny), str(cfg.nz), str(cfg.nphases), str(tmp / "state.dat"),
+                    str(tmp / "flags.bin"), str(tmp / "grains.bin"), str(tmp / "out")], check=True)
+    return {"dat": (tmp / "out_full.dat").read_bytes(), "grains": (tmp / "out.grains").read_bytes(),
+            "vtk": (tmp / "out.vts").read_bytes()}
+
+
+GRAIN_RECORD = np.dtype([("exist", "u1"), ("mobile", "u1"), ("stage", "<i4"), ("parent", "<u8"), ("phase", "<u8"),
+                         ("variant", "<u8"), ("state", "<i4"), ("density", "<f8"), ("refvolume", "<f8"),
+                         ("volume", "<f8"), ("maxvolume", "<f8"), ("kinematics", "<f8", (30,)), ("q", "<f8", (4,))])
+
+
+def parse_dat(b: bytes):
+    """(dims, counts, indices, values) of a raw phase-field file."""
+    dims = np.frombuffer(b[:12], np.int32)
+    body = np.frombuffer(b[12:], np.uint64)
+    counts, idx, val, pos = [], [], [], 0
+    while pos < body.size:
+        k = int(body[pos])
+        counts.append(k)
+        blk = body[pos + 1: pos + 1 + 2 * k]
+        idx.append(blk[0::2].astype(np.int64))
+        val.append(blk[1::2].view(np.float64))
+        pos += 1 + 2 * k
+    return dims, np.array(counts), np.concatenate(idx), np.concatenate(val)
+
+
+def assert_files_close(got: dict, exp: dict, rtol: float):
+    """Same structure and integers; floating-point values within rtol (relative to max(1, |x|))."""
+    gd, ed = parse_dat(got["dat"]), parse_dat(exp["dat"])
+    for a, b in zip(gd[:3], ed[:3]):
+        assert np.array_equal(a, b), "phase-field file: structure differs"
+    assert np.abs(gd[3] - ed[3]).max() <= rtol
+    n = np.frombuffer(got["grains"][:8], np.uint64)[0]
+    assert n == np.frombuffer(exp["grains"][:8], np.uint64)[0]
+    gg = np.frombuffer(got["grains"][8:], GRAIN_RECORD)
+    eg = np.frombuffer(exp["grains"][8:], GRAIN_RECORD)
+    for f in GRAIN_RECORD.names:
+        if f in ("volume", "maxvolume"):
+            assert np.all(np.abs(gg[f] - eg[f]) <= rtol * np.maximum(1.0, np.abs(eg[f]))), f"grain {f}"
+        else:
+            assert np.array_equal(gg[f], eg[f]), f"grain {f}"
+    gl, el = got["vtk"].decode().splitlines(), exp["vtk"].decode().splitlines()
+    assert len(gl) == len(el)
+    for a, b in zip(gl, el):
+        if a == b:
+            continue
+        x, y = float(a), float(b)
+        assert abs(x - y) <= rtol * max(1.0, abs(y)), f"VTK value {a} vs {b}"

@@ -9,7 +9,8 @@ from openphase_amd.engine import StepConfig
 from openphase_amd.microstructure import Lists, quaternion_to_matrix, storage_shape
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-CASES = sorted(p.stem for p in GOLDEN.glob("*.npz"))
+CASES = sorted(p.stem for p in GOLDEN.glob("*.npz") if not p.stem.startswith("io_"))
+IO_CASES = sorted(p.stem for p in GOLDEN.glob("io_*.npz"))
 
 
 class Case:

@@ -2,7 +2,8 @@
 
 Runs oracle/_ref/ref_driver (built from the reference sources by oracle/Makefile.ref) on small
 seeded microstructures and stores, per case, the input state, the parameters and the finalized
-state (lists, grain volumes, volume ratios) at a few steps. The tests compare the CPU oracle and
+state (lists, grain volumes, volume ratios) at a few steps; the io_* files hold the bytes of the
+reference's own output files (raw phase field, grain table, VTK) after a few steps. The tests compare the CPU oracle and
 the GPU path against these files; the reference itself is never needed at test time.
 
     python tests/golden/make_golden.py [case ...]
@@ -73,6 +74,45 @@ def cases():
     return out
 
 
+def io_cases():
+    """Output-format fixtures: (name, cfg, ngrains, seed, grain_phase, quaternions, step)."""
+    q, _ = random_rotations(6, 21)
+    aniso = StepConfig(20, 16, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                       energy_model=np.ones((2, 2), np.int32), epsilon=np.full((2, 2), 0.05))
+    return [
+        ("io_gg2d_24", StepConfig(24, 20, 0), 6, 20, None, None, 4),
+        ("io_aniso2d_20", aniso, 6, 21, (np.arange(6) % 2).astype(np.int32), q, 3),
+        ("io_gg3d_10", StepConfig(10, 9, 8), 5, 22, None, None, 2),
+    ]
+
+
+def main_io(only=()):
+    """The reference's own writers (PhaseField::Write, GrainsProperties::Write, PhaseField::WriteVTK)
+    after a few steps; the tests feed the same state to include/openphase_amd/io.h."""
+    for name, cfg, ng, seed, phase, quat, step in io_cases():
+        if only and name not in only:
+            continue
+        lists = voronoi_grains(cfg.nx, cfg.ny, cfg.nz, ng, seed=seed)
+        phase = np.zeros(ng, np.int32) if phase is None else phase
+        dt = cfg.max_time_step()
+        with tempfile.TemporaryDirectory() as td:
+            _, ref = O.run_reference(td, cfg, lists, phase, quat, step, dt, [step], write_files=True)
+        files = ref[step][3]
+        rec = dict(
+            nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, nphases=cfg.nphases, sigma=np.asarray(cfg.sigma, np.float64),
+            mu=np.asarray(cfg.mu, np.float64),
+            energy_model=np.zeros((cfg.nphases, cfg.nphases), np.int32) if cfg.energy_model is None
+            else np.asarray(cfg.energy_model, np.int32),
+            epsilon=np.zeros((cfg.nphases, cfg.nphases)) if cfg.epsilon is None else np.asarray(cfg.epsilon),
+            dt=dt, step=step, grain_phase=phase, quaternions=np.zeros((ng, 4)) if quat is None else quat,
+            init_count=lists.count, init_index=lists.index, init_value=lists.value,
+            dat=np.frombuffer(files["dat"], np.uint8), grains=np.frombuffer(files["grains"], np.uint8),
+            vtk=np.frombuffer(files["vtk"], np.uint8),
+        )
+        np.savez_compressed(HERE / f"{name}.npz", **rec)
+        print(name, "dat", len(files["dat"]), "grains", len(files["grains"]), "vtk", len(files["vtk"]))
+
+
 def main(only=()):
     for name, cfg, ng, seed, phase, quat, nsteps, dumps, dtf in cases():
         if only and name not in only:
@@ -111,3 +151,4 @@ def main(only=()):
 
 if __name__ == "__main__":
     main(tuple(sys.argv[1:]))
+    main_io(tuple(sys.argv[1:]))
