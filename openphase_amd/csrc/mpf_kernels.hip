@@ -646,17 +646,15 @@ __global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A0, cons
 
 // Cells the main kernel deferred, with register-resident lists. LEVEL 0 (list 0: merged lists of
 // 5-6 entries, more than 4 own entries, or 2-4 entries next to a grain below its reference
-// volume) computes 2-6 entries with capacity 8; LEVEL 1 (list 1) computes 7 and 8; LEVEL 2
-// (list 2, capacity 12) computes 9 and 10. Anything else moves on to the next list; list 3 belongs
-// to the 16-entry kernel. Splitting the sizes keeps the register count of the common level low
-// and the few long lists off the private-memory path.
+// volume) computes 2-6 entries with capacity 8 and sends 7-8 entries to list 1 and longer lists
+// (capacity 8 exceeded) to list 2; LEVEL 1 computes 7 and 8 entries, LEVEL 2 (capacity 12) 9 and
+// 10; both send anything else to list 3, the 16-entry kernel's. Lists 1 and 2 are independent, so
+// one launch runs both levels (mid_tail_kernel) and their few long cells overlap. Splitting the
+// sizes keeps the register count of the common level low and the few long lists off the
+// private-memory path.
 template <typename TV, typename TC, bool GRAD, bool DG, int LEVEL>
-__global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restrict__ batch, int32_t ndom)
+__device__ __forceinline__ void mid_level(const StepArgs& A, unsigned lblock, unsigned per)
 {
-    // gridDim.x / ndom blocks per domain, each domain's list walked grid-stride by its blocks
-    const unsigned per = gridDim.x / static_cast<unsigned>(ndom);
-    const unsigned dom = blockIdx.x / per, lblock = blockIdx.x - dom * per;
-    const StepArgs& A = batch[dom];
     constexpr int K = LEVEL < 2 ? 8 : 12;
     const unsigned n = A.work_count[LEVEL];
     const uint32_t* list = A.work_list + LEVEL * A.ncells;
@@ -670,7 +668,8 @@ __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restr
         for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
         if (valid) {
             const int64_t c = list[i];
-            int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
+            const int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
+            int dest = -1;  // work list the cell moves on to
             if (r == kCellWide) {
                 if (LEVEL == 0) {
                     switch (L.ne) {
@@ -679,27 +678,29 @@ __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restr
                     case 4: compute_cell<4, true, K, TC, GRAD, DG>(A, L, empty, c); break;
                     case 5: compute_cell<5, true, K, TC, GRAD, DG>(A, L, empty, c); break;
                     case 6: compute_cell<6, true, K, TC, GRAD, DG>(A, L, empty, c); break;
-                    default: r = kCellDeferred; break;
+                    default: dest = 1; break;
                     }
                 } else if (LEVEL == 1) {
                     switch (L.ne) {
                     case 7: compute_cell<7, true, K, TC, GRAD, DG>(A, L, empty, c); break;
                     case 8: compute_cell<8, true, K, TC, GRAD, DG>(A, L, empty, c); break;
-                    default: r = kCellDeferred; break;
+                    default: dest = 3; break;
                     }
                 } else {
                     switch (L.ne) {
                     case 9: compute_cell<9, true, K, TC, GRAD, DG>(A, L, empty, c); break;
                     case 10: compute_cell<10, true, K, TC, GRAD, DG>(A, L, empty, c); break;
-                    default: r = kCellDeferred; break;
+                    default: dest = 3; break;
                     }
                 }
+            } else if (r == kCellDeferred) {
+                dest = LEVEL == 0 ? 2 : 3;  // more entries than this level's capacity
             }
-            if (r == kCellDeferred) {
+            if (dest >= 0) {
 #pragma unroll
                 for (int t = 0; t < K; ++t) L.keep[t] = false;
-                const unsigned slot = atomicAdd(&A.work_count[LEVEL + 1], 1u);
-                A.work_list[(LEVEL + 1) * A.ncells + slot] = static_cast<uint32_t>(c);
+                const unsigned slot = atomicAdd(&A.work_count[dest], 1u);
+                A.work_list[dest * A.ncells + slot] = static_cast<uint32_t>(c);
             } else if (!store_cell<K, TV, TC, GRAD>(A, c, L)) {
                 report_overflow<K, TC, GRAD>(A, c, L);
             }
@@ -709,6 +710,26 @@ __global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restr
         for (int t = 0; t < K; ++t) wave_volume_add<true>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
         if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
     }
+}
+
+// gridDim.x / ndom blocks per domain, each domain's list walked grid-stride by its blocks
+template <typename TV, typename TC, bool GRAD, bool DG, int LEVEL>
+__global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restrict__ batch, int32_t ndom)
+{
+    const unsigned per = gridDim.x / static_cast<unsigned>(ndom);
+    const unsigned dom = blockIdx.x / per, lblock = blockIdx.x - dom * per;
+    mid_level<TV, TC, GRAD, DG, LEVEL>(batch[dom], lblock, per);
+}
+
+// Levels 1 and 2 in one launch: the first half of each domain's blocks walks list 1, the second
+// half list 2.
+template <typename TV, typename TC, bool GRAD, bool DG>
+__global__ __launch_bounds__(kWideBlock) void mid_tail_kernel(const StepArgs* __restrict__ batch, int32_t ndom)
+{
+    const unsigned per = gridDim.x / static_cast<unsigned>(ndom);
+    const unsigned dom = blockIdx.x / per, lblock = blockIdx.x - dom * per, half = per / 2;
+    if (lblock < half) mid_level<TV, TC, GRAD, DG, 1>(batch[dom], lblock, half);
+    else mid_level<TV, TC, GRAD, DG, 2>(batch[dom], lblock - half, per - half);
 }
 
 // Cells whose merged list is longer than 8 entries (up to 16, held in private memory).
@@ -838,7 +859,7 @@ __global__ void grains_kernel(const GrainArgs* __restrict__ batch, const uint32_
     const int g = static_cast<int>(blockIdx.x - first[dom]) * blockDim.x + threadIdx.x;
     if (g == 0) {
         G.stats[0] += G.work_count[0];
-        G.stats[1] += G.work_count[1];
+        G.stats[1] += G.work_count[1] + G.work_count[2];
         G.work_count[0] = 0u;
         G.work_count[1] = 0u;
         G.work_count[2] = 0u;
@@ -916,10 +937,10 @@ int launch_deferred(const StepArgs* args, int ndom, int per_domain, bool fp32, b
 {
     const dim3 grid(static_cast<unsigned>(per_domain) * static_cast<unsigned>(ndom));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid2(2u * static_cast<unsigned>(per_domain) * static_cast<unsigned>(ndom));
 #define MPF_DEFERRED(TV, F)                                                                              \
     hipLaunchKernelGGL((mid_kernel<TV, TV, F, F, 0>), grid, dim3(kWideBlock), 0, s, args, ndom);    \
-    hipLaunchKernelGGL((mid_kernel<TV, TV, F, F, 1>), grid, dim3(kWideBlock), 0, s, args, ndom);    \
-    hipLaunchKernelGGL((mid_kernel<TV, TV, F, F, 2>), grid, dim3(kWideBlock), 0, s, args, ndom);    \
+    hipLaunchKernelGGL((mid_tail_kernel<TV, TV, F, F>), grid2, dim3(kWideBlock), 0, s, args, ndom); \
     hipLaunchKernelGGL((wide_kernel<TV, TV, F, F>), grid, dim3(kWideBlock), 0, s, args, ndom);
     if (fp32) {
         if (full) { MPF_DEFERRED(float, true) } else { MPF_DEFERRED(float, false) }
