@@ -626,8 +626,11 @@ __device__ __forceinline__ void step_block(const StepArgs& A, int64_t base, uint
 // One domain (ndom <= 1): arguments by value in the kernel-argument segment, which the hardware
 // preloads into scalar registers. Several domains in one launch (ensembles): the argument table
 // and block prefix sums in device memory; blocks are XCD-ordered over the whole launch first.
+// Register budget: 4 wavefronts per SIMD (128 VGPRs, a few spilled) for the plain variants, 3
+// (168 VGPRs) for the orientation / driving-force ones, which spill heavily at 128; measured at
+// 256^3: 3.68 ms at 4 wavefronts vs 4.58 ms at 3 and 4.36 ms at 5 (plain FP64).
 template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
-__global__ __launch_bounds__(kBlock, 4) void step_kernel(const StepArgs A0, const StepArgs* __restrict__ batch,
+__global__ __launch_bounds__(kBlock, (GRAD || DG) ? 3 : 4) void step_kernel(const StepArgs A0, const StepArgs* __restrict__ batch,
                                                           const uint32_t* __restrict__ first, int32_t ndom)
 {
     __shared__ uint32_t s_codes[7][kBlock];
@@ -712,9 +715,11 @@ __device__ __forceinline__ void mid_level(const StepArgs& A, unsigned lblock, un
     }
 }
 
-// gridDim.x / ndom blocks per domain, each domain's list walked grid-stride by its blocks
+// gridDim.x / ndom blocks per domain, each domain's list walked grid-stride by its blocks. The
+// plain variants run at 3 wavefronts per SIMD (168 VGPRs, no spill; level 0 at 256^3: 0.60 ms vs
+// 0.71 ms at the unconstrained 219 VGPRs and 2 wavefronts).
 template <typename TV, typename TC, bool GRAD, bool DG, int LEVEL>
-__global__ __launch_bounds__(kWideBlock) void mid_kernel(const StepArgs* __restrict__ batch, int32_t ndom)
+__global__ __launch_bounds__(kWideBlock, (GRAD || DG) ? 1 : 3) void mid_kernel(const StepArgs* __restrict__ batch, int32_t ndom)
 {
     const unsigned per = gridDim.x / static_cast<unsigned>(ndom);
     const unsigned dom = blockIdx.x / per, lblock = blockIdx.x - dom * per;
