@@ -228,6 +228,24 @@ __device__ __forceinline__ uint8_t box_code(const uint32_t codes[7], int k)
     return static_cast<uint8_t>(codes[k >> 2] >> (8 * (k & 3)));
 }
 
+// box_code / Box::rows for an index known only at run time: selects over the registers
+__device__ __forceinline__ uint8_t box_code_dyn(const uint32_t codes[7], int k)
+{
+    const int w = k >> 2;
+    uint32_t v = codes[0];
+#pragma unroll
+    for (int t = 1; t < 7; ++t) v = w == t ? codes[t] : v;
+    return static_cast<uint8_t>(v >> (8 * (k & 3)));
+}
+
+__device__ __forceinline__ uint32_t box_row_dyn(const Box& B, int r)
+{
+    uint32_t v = B.rows[0];
+#pragma unroll
+    for (int t = 1; t < 9; ++t) v = r == t ? B.rows[t] : v;
+    return v;
+}
+
 // Count bytes of the whole box (27 independent loads, packed four to a word; inactive positions
 // read as 0). Returns wide_interface(): flag 2 (more than one entry) or a flag-2 cell in the box.
 template <typename TV>
@@ -350,64 +368,133 @@ __device__ __forceinline__ int merge_neighbours(const StepArgs& A, const Box& B,
     const int ip_end = A.dnx ? 3 : 2;
     // Neighbour values are read as stored: FP32 lists store every value, the implicit entry's as
     // well (only the cell's own update recovers it as 1 - sum of the others in FP64).
+    if constexpr (GRAD) {
+        // orientation-dependent variants: a plane of nine neighbours per batch of loads
 #pragma unroll 1
-    for (int ip = A.dnx ? 0 : 1; ip < ip_end; ++ip) {
-        // the plane's count bytes and row starts (selects: ip is a loop variable)
-        uint8_t pc[9];
+        for (int ip = A.dnx ? 0 : 1; ip < ip_end; ++ip) {
+            // the plane's count bytes and row starts (selects: ip is a loop variable)
+            uint8_t pc[9];
 #pragma unroll
-        for (int j = 0; j < 9; ++j)
-            pc[j] = ip == 0 ? box_code(codes, j) : (ip == 1 ? box_code(codes, 9 + j) : box_code(codes, 18 + j));
-        uint32_t prow[3];
+            for (int j = 0; j < 9; ++j)
+                pc[j] = ip == 0 ? box_code(codes, j) : (ip == 1 ? box_code(codes, 9 + j) : box_code(codes, 18 + j));
+            uint32_t prow[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) prow[r] = ip == 0 ? B.rows[r] : (ip == 1 ? B.rows[3 + r] : B.rows[6 + r]);
-        TV v0[9], v1[9];
-        uint32_t g0[9], g1[9];
+            for (int r = 0; r < 3; ++r) prow[r] = ip == 0 ? B.rows[r] : (ip == 1 ? B.rows[3 + r] : B.rows[6 + r]);
+            TV v0[9], v1[9];
+            uint32_t g0[9], g1[9];
 #pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const int m = ListCode<TV>::count(pc[j]), imp = ListCode<TV>::implicit(pc[j]);
-            const uint32_t nb = prow[j / 3] + B.zs[j % 3];
-            // a lone implicit entry (FP32) has no stored value: it is 1
-            const bool lone = ListCode<TV>::kImplicit && m == 1 && imp == 0;
-            g0[j] = buffer_load<uint16_t>(ri0, m > 0 ? nb * 2u : kNoLoad);
-            v0[j] = lone ? TV(1) : buffer_load<TV>(rv0, m > 0 ? nb * SV : kNoLoad);
-            g1[j] = buffer_load<uint16_t>(ri1, m > 1 ? nb * 2u : kNoLoad);
-            v1[j] = buffer_load<TV>(rv1, m > 1 ? nb * SV : kNoLoad);
-        }
-        const int dx = ip - 1;
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const int jj = j / 3 - 1, kk = j % 3 - 1;
-            const int m = ListCode<TV>::count(pc[j]);
-            const uint32_t nb = prow[j / 3] + B.zs[j % 3];
-            const int nj = (jj != 0) + (kk != 0);
-            const bool side = dx != 0 || nj != 0;
-            const TC w = static_cast<TC>(dx != 0 ? A.wl[nj + 1] : A.wl[nj]);
-            const TC gw = static_cast<TC>(dx != 0 ? A.wg[nj + 1] : A.wg[nj]);
-            const TC gwx = dx == 0 ? TC(0) : (dx < 0 ? -gw : gw);
-            const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
-            const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
-            // entries beyond the second (rare), loaded on demand
-            TV v2 = TV(0);
-            uint32_t g2 = 0u;
-            if (m > 2) {
-                g2 = A.id_in[2 * N + nb];
-                v2 = load_val<TV>(A.val_in, 2 * N + nb);
+            for (int j = 0; j < 9; ++j) {
+                const int m = ListCode<TV>::count(pc[j]), imp = ListCode<TV>::implicit(pc[j]);
+                const uint32_t nb = prow[j / 3] + B.zs[j % 3];
+                // a lone implicit entry (FP32) has no stored value: it is 1
+                const bool lone = ListCode<TV>::kImplicit && m == 1 && imp == 0;
+                g0[j] = buffer_load<uint16_t>(ri0, m > 0 ? nb * 2u : kNoLoad);
+                v0[j] = lone ? TV(1) : buffer_load<TV>(rv0, m > 0 ? nb * SV : kNoLoad);
+                g1[j] = buffer_load<uint16_t>(ri1, m > 1 ? nb * 2u : kNoLoad);
+                v1[j] = buffer_load<TV>(rv1, m > 1 ? nb * SV : kNoLoad);
             }
-            const TC a0 = static_cast<TC>(v0[j]), a1 = static_cast<TC>(v1[j]);
-            merge_entry<K, TC, GRAD>(L, ne, over, m > 0 && a0 != TC(0), g0[j], w * a0,
-                                     gwx * a0, gwy * a0, gwz * a0, side);
-            merge_entry<K, TC, GRAD>(L, ne, over, m > 1 && a1 != TC(0), g1[j], w * a1,
-                                     gwx * a1, gwy * a1, gwz * a1, side);
-            if (m > 2) {
-                const TC a2 = static_cast<TC>(v2);
-                merge_entry<K, TC, GRAD>(L, ne, over, a2 != TC(0), g2, w * a2, gwx * a2, gwy * a2, gwz * a2, side);
-                if (!MANY) {
-                    over = over || m > 3;
-                } else {
-                    for (int q = 3; q < m; ++q) {
-                        const TC aq = static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
-                        const uint32_t gq = A.id_in[static_cast<int64_t>(q) * N + nb];
-                        merge_entry<K, TC, GRAD>(L, ne, over, aq != TC(0), gq, w * aq, gwx * aq, gwy * aq, gwz * aq, side);
+            const int dx = ip - 1;
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const int jj = j / 3 - 1, kk = j % 3 - 1;
+                const int m = ListCode<TV>::count(pc[j]);
+                const uint32_t nb = prow[j / 3] + B.zs[j % 3];
+                const int nj = (jj != 0) + (kk != 0);
+                const bool side = dx != 0 || nj != 0;
+                const TC w = static_cast<TC>(dx != 0 ? A.wl[nj + 1] : A.wl[nj]);
+                const TC gw = static_cast<TC>(dx != 0 ? A.wg[nj + 1] : A.wg[nj]);
+                const TC gwx = dx == 0 ? TC(0) : (dx < 0 ? -gw : gw);
+                const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
+                const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
+                // entries beyond the second (rare), loaded on demand
+                TV v2 = TV(0);
+                uint32_t g2 = 0u;
+                if (m > 2) {
+                    g2 = A.id_in[2 * N + nb];
+                    v2 = load_val<TV>(A.val_in, 2 * N + nb);
+                }
+                const TC a0 = static_cast<TC>(v0[j]), a1 = static_cast<TC>(v1[j]);
+                merge_entry<K, TC, GRAD>(L, ne, over, m > 0 && a0 != TC(0), g0[j], w * a0,
+                                         gwx * a0, gwy * a0, gwz * a0, side);
+                merge_entry<K, TC, GRAD>(L, ne, over, m > 1 && a1 != TC(0), g1[j], w * a1,
+                                         gwx * a1, gwy * a1, gwz * a1, side);
+                if (m > 2) {
+                    const TC a2 = static_cast<TC>(v2);
+                    merge_entry<K, TC, GRAD>(L, ne, over, a2 != TC(0), g2, w * a2, gwx * a2, gwy * a2, gwz * a2, side);
+                    if (!MANY) {
+                        over = over || m > 3;
+                    } else {
+                        for (int q = 3; q < m; ++q) {
+                            const TC aq = static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
+                            const uint32_t gq = A.id_in[static_cast<int64_t>(q) * N + nb];
+                            merge_entry<K, TC, GRAD>(L, ne, over, aq != TC(0), gq, w * aq, gwx * aq, gwy * aq, gwz * aq, side);
+                        }
+                    }
+                }
+            }
+        }
+    } else {
+        // plain variants: a row of three neighbours per batch of loads, which keeps the step kernel
+        // within 128 VGPRs without spilling (256^3 FP64 step kernel 3.26 ms vs 3.66 ms with plane
+        // batches; the orientation variants lose about 6% this way)
+#pragma unroll 1
+        for (int ir = (A.dnx ? 0 : 3); ir < ip_end * 3; ++ir) {
+            // one row of three neighbours (plane ip, row r) per batch of loads
+            const int ip = ir / 3, r = ir - 3 * (ir / 3);
+            const int b = ir * 3;  // box index of the row's first neighbour
+            uint8_t pc[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) pc[j] = box_code_dyn(codes, b + j);
+            const uint32_t prow = box_row_dyn(B, ir);
+            TV v0[3], v1[3];
+            uint32_t g0[3], g1[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int m = ListCode<TV>::count(pc[j]), imp = ListCode<TV>::implicit(pc[j]);
+                const uint32_t nb = prow + B.zs[j];
+                // a lone implicit entry (FP32) has no stored value: it is 1
+                const bool lone = ListCode<TV>::kImplicit && m == 1 && imp == 0;
+                g0[j] = buffer_load<uint16_t>(ri0, m > 0 ? nb * 2u : kNoLoad);
+                v0[j] = lone ? TV(1) : buffer_load<TV>(rv0, m > 0 ? nb * SV : kNoLoad);
+                g1[j] = buffer_load<uint16_t>(ri1, m > 1 ? nb * 2u : kNoLoad);
+                v1[j] = buffer_load<TV>(rv1, m > 1 ? nb * SV : kNoLoad);
+            }
+            const int dx = ip - 1, jj = r - 1;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int kk = j - 1;
+                const int m = ListCode<TV>::count(pc[j]);
+                const uint32_t nb = prow + B.zs[j];
+                const int nj = (jj != 0) + (kk != 0);
+                const bool side = dx != 0 || nj != 0;
+                const TC w = static_cast<TC>(dx != 0 ? A.wl[nj + 1] : A.wl[nj]);
+                const TC gw = static_cast<TC>(dx != 0 ? A.wg[nj + 1] : A.wg[nj]);
+                const TC gwx = dx == 0 ? TC(0) : (dx < 0 ? -gw : gw);
+                const TC gwy = jj == 0 ? TC(0) : (jj < 0 ? -gw : gw);
+                const TC gwz = kk == 0 ? TC(0) : (kk < 0 ? -gw : gw);
+                // entries beyond the second (rare), loaded on demand
+                TV v2 = TV(0);
+                uint32_t g2 = 0u;
+                if (m > 2) {
+                    g2 = A.id_in[2 * N + nb];
+                    v2 = load_val<TV>(A.val_in, 2 * N + nb);
+                }
+                const TC a0 = static_cast<TC>(v0[j]), a1 = static_cast<TC>(v1[j]);
+                merge_entry<K, TC, GRAD>(L, ne, over, m > 0 && a0 != TC(0), g0[j], w * a0,
+                                         gwx * a0, gwy * a0, gwz * a0, side);
+                merge_entry<K, TC, GRAD>(L, ne, over, m > 1 && a1 != TC(0), g1[j], w * a1,
+                                         gwx * a1, gwy * a1, gwz * a1, side);
+                if (m > 2) {
+                    const TC a2 = static_cast<TC>(v2);
+                    merge_entry<K, TC, GRAD>(L, ne, over, a2 != TC(0), g2, w * a2, gwx * a2, gwy * a2, gwz * a2, side);
+                    if (!MANY) {
+                        over = over || m > 3;
+                    } else {
+                        for (int q = 3; q < m; ++q) {
+                            const TC aq = static_cast<TC>(load_val<TV>(A.val_in, static_cast<int64_t>(q) * N + nb));
+                            const uint32_t gq = A.id_in[static_cast<int64_t>(q) * N + nb];
+                            merge_entry<K, TC, GRAD>(L, ne, over, aq != TC(0), gq, w * aq, gwx * aq, gwy * aq, gwz * aq, side);
+                        }
                     }
                 }
             }
