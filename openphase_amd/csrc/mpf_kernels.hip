@@ -626,11 +626,12 @@ __device__ __forceinline__ void step_block(const StepArgs& A, int64_t base, uint
 // One domain (ndom <= 1): arguments by value in the kernel-argument segment, which the hardware
 // preloads into scalar registers. Several domains in one launch (ensembles): the argument table
 // and block prefix sums in device memory; blocks are XCD-ordered over the whole launch first.
-// Register budget: 4 wavefronts per SIMD (128 VGPRs, a few spilled) for the plain variants, 3
-// (168 VGPRs) for the orientation / driving-force ones, which spill heavily at 128; measured at
-// 256^3: 3.68 ms at 4 wavefronts vs 4.58 ms at 3 and 4.36 ms at 5 (plain FP64).
+// Register budget: 6 wavefronts per SIMD (80 VGPRs, some spilled) for the plain variants, 3
+// (168 VGPRs) for the orientation / driving-force ones, which spill heavily below that. Plain
+// step kernel at 256^3 (FP64 / FP32): 3.25 / 2.79 ms at 5 wavefronts (87 VGPRs, no spill),
+// 2.94 / 2.45 ms at 6, 3.27 / 2.54 ms at 7, 4.18 / 2.87 ms at 8: latency hiding outweighs spills.
 template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
-__global__ __launch_bounds__(kBlock, (GRAD || DG) ? 3 : 4) void step_kernel(const StepArgs A0, const StepArgs* __restrict__ batch,
+__global__ __launch_bounds__(kBlock, (GRAD || DG) ? 3 : 6) void step_kernel(const StepArgs A0, const StepArgs* __restrict__ batch,
                                                           const uint32_t* __restrict__ first, int32_t ndom)
 {
     __shared__ uint32_t s_codes[7][kBlock];
