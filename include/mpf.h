@@ -208,8 +208,8 @@ mpf_status mpf_restore_state(mpf_ctx* ctx, const void* device_src);
 mpf_status mpf_adjoint_step(mpf_ctx* ctx, double dt, const double* lam_next, double* lam_prev, double* grad);
 
 /* Launch-shape autotuner (no reference counterpart): times `steps` steps of each candidate launch
- * shape - XCD-contiguous block order of the step kernel on/off, 256 / 1024 / 4096 blocks for the
- * deferred-cell kernels - on the context's own state (saved and restored around every trial, so
+ * shape - XCD-contiguous block order of the step kernel on/off, 256 / 1024 / 4096 / 16384 blocks
+ * for the deferred-cell kernels - on the context's own state (saved and restored around every trial, so
  * the state is unchanged afterwards) and keeps the fastest. Results do not depend on the shape.
  * The chosen shape is returned through the optional out pointers. */
 mpf_status mpf_autotune(mpf_ctx* ctx, double dt, int32_t steps, int32_t* xcd_order, int32_t* deferred_grid);

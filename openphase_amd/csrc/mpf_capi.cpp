@@ -361,6 +361,9 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         return bail(MPF_ERR_UNSUPPORTED, "at most 2^28 - 1 cells per device (e.g. 645^3)");
     c->slots = grid->slots;
     c->fp32 = grid->precision == MPF_FP32;
+    // deferred-cell grid: 16384 blocks from 4M cells on (256^3 FP64 step 3.98 ms at 4096 blocks,
+    // 3.92 at 8192, 3.85 at 16384: the level-0 list then needs one pass), 4096 below
+    c->deferred_grid = c->ncells >= (int64_t{1} << 22) ? 4 * mpf::kWideGrid : mpf::kWideGrid;
     c->nphases = params->nphases;
     const size_t np2 = static_cast<size_t>(c->nphases) * c->nphases;
     c->sigma.assign(params->sigma, params->sigma + np2);
@@ -1065,11 +1068,11 @@ mpf_status mpf_autotune(mpf_ctx* c, double dt, int32_t steps, int32_t* xcd_order
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!st) st = c->hip(hipEventCreate(&e0), "hipEventCreate");
     if (!st) st = c->hip(hipEventCreate(&e1), "hipEventCreate");
-    const int grids[3] = {256, 1024, mpf::kWideGrid};
+    const int grids[4] = {256, 1024, mpf::kWideGrid, 4 * mpf::kWideGrid};
     float best = 1e30f;
     int best_x = c->xcd_order, best_g = c->deferred_grid;
     for (int x = 0; x < 2 && !st; ++x)
-        for (int gi = 0; gi < 3 && !st; ++gi) {
+        for (int gi = 0; gi < 4 && !st; ++gi) {
             c->xcd_order = x;
             c->deferred_grid = grids[gi];
             c->cur = cur0;
