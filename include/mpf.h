@@ -149,8 +149,11 @@ mpf_status mpf_get_grains(mpf_ctx* ctx, double* volume, double* volume_ratio);
  * besides Phase and Orientation, which the caller set. */
 mpf_status mpf_get_grain_records(mpf_ctx* ctx, double* maxvolume, double* refvolume, uint8_t* exist, int32_t* stage);
 
-/* InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556) evaluated from
- * the phase-pair maxima of energy/mobility (ISO models). */
+/* InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556): c dx^2 /
+ * (maxMobility maxEnergy RegularizationFactor) from the largest interface energy and mobility the
+ * last step's IP.Set evaluated over the pairs of its interface cells (SetSR, :603-756; CUBIC
+ * energies as computed from the orientations). DBL_MAX before the first step, as the reference
+ * before its first IP.Set. */
 mpf_status mpf_max_time_step(mpf_ctx* ctx, double* dt_max);
 
 /* Number of limiter plausibility warnings (PhaseField.cpp:1536-1561) since creation. */
@@ -224,6 +227,9 @@ mpf_status mpf_set_launch_shape(mpf_ctx* ctx, int32_t xcd_order, int32_t deferre
 typedef struct mpf_ensemble mpf_ensemble;
 mpf_status mpf_ensemble_create(mpf_ctx* const* members, int32_t n, mpf_ensemble** out);
 mpf_status mpf_ensemble_step(mpf_ensemble* ens, const double* dt, int32_t nsteps);
+/* Description of the last error of the ensemble itself (graph capture / launch); member errors
+ * are reported by mpf_error_message of the member. */
+const char* mpf_ensemble_error_message(mpf_ensemble* ens);
 void mpf_ensemble_destroy(mpf_ensemble* ens);
 
 /* Human-readable description of the last error on this context (static storage per context). */

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cfloat>
 #include <cstdint>
 #include <fstream>
 #include <map>
@@ -153,6 +154,34 @@ class Settings {
     std::vector<std::string> PhaseNames;
     std::vector<uint8_t> Solid;
     std::string InputFileName;
+};
+
+// ------------------------------------------------------------------------------------------
+// RunTimeControl (include/RunTimeControl.h; keys of RunTimeControl::ReadInput,
+// src/RunTimeControl.cpp:99-131): the time loop's step counter, number of steps and time step.
+class RunTimeControl {
+ public:
+    RunTimeControl() = default;
+    RunTimeControl(Settings& s, const std::string& file) { (void)s; ReadInput(file); }
+    void ReadInput(const std::string& file)
+    {
+        InputFile in(file);
+        const char* m = "RunTimeControl";
+        nSteps = in.has(m, "MaxTimeStep") ? in.geti(m, "MaxTimeStep", 0) : in.geti(m, "nSteps", 0, true);
+        dt = in.getd(m, "dt", 0.0, true);
+        tStart = in.has(m, "StartTimeStep") ? in.geti(m, "StartTimeStep", 0) : in.geti(m, "tStart", 0);
+        tStep = tStart;
+        FTime = in.has(m, "VTKOutputInterval") ? in.geti(m, "VTKOutputInterval", 1) : in.geti(m, "FTime", 1);
+        STime = in.has(m, "ConsoleOutputInterval") ? in.geti(m, "ConsoleOutputInterval", 1) : in.geti(m, "STime", 1);
+        RTime = in.has(m, "CheckpointInterval") ? in.geti(m, "CheckpointInterval", 1) : in.geti(m, "tRstrt", 1);
+    }
+    void IncrementTimeStep() { ++tStep; }
+    bool WriteVTK() const { return FTime > 0 && tStep % FTime == 0; }
+    bool WriteToScreen() const { return STime > 0 && tStep % STime == 0; }
+    bool WriteRawData() const { return RTime > 0 && tStep % RTime == 0; }
+    int tStep = 0, tStart = 0, nSteps = 0;
+    int FTime = 1, STime = 1, RTime = 1;
+    double dt = 0.0;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -432,18 +461,18 @@ class InterfaceProperties {
         Phase.check_bc(BC);
         Phase.ensure_context();
         Phase.stage_ = 1;
+        phase_ = &Phase;
     }
-    // InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556)
+    // InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556): from the
+    // largest energy and mobility the pairs of the interface cells had in the last IP.Set
+    // (SetSR, :748-756), DBL_MAX before the first. The maxima are taken when the step runs on the
+    // device (in PhaseField::MergeIncrements), so between IP.Set and MergeIncrements this still
+    // reports the previous step's.
     double ReportMaximumTimeStep() const
     {
-        double me = 0.0, mm = 0.0;
-        for (size_t k = 0; k < Sigma.size(); ++k) {
-            const double maxe = EnergyModel[k] == MPF_ENERGY_CUBIC ? Sigma[k] * (1.0 + Epsilon[k] * 1.41) : Sigma[k];
-            me = std::max(me, maxe);
-            mm = std::max(mm, Mu[k]);
-        }
-        const double f = active_ == 1 ? 0.50 : (active_ == 2 ? 0.25 : 0.16);
-        return f * dx_ * dx_ / (mm * me * RegularizationFactor);
+        double dt = DBL_MAX;
+        if (phase_ && phase_->ctx_) check(mpf_max_time_step(phase_->ctx_, &dt), phase_->ctx_, "ReportMaximumTimeStep");
+        return dt;
     }
     size_t Nphases;
     std::vector<double> Sigma, Mu, Epsilon;
@@ -457,6 +486,7 @@ class InterfaceProperties {
     friend class PhaseField;
     double dx_;
     int active_;
+    const PhaseField* phase_ = nullptr;  // the phase field of the last Set
 };
 
 // ------------------------------------------------------------------------------------------

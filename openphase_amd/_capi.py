@@ -13,6 +13,8 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libmpf_hip.so"
 
 MPF_OK = 0
+MPF_ERR_ARG, MPF_ERR_UNSUPPORTED, MPF_ERR_HIP, MPF_ERR_NOMEM = 1, 2, 3, 4
+MPF_SLOT_OVERFLOW, MPF_EMPTY_NODE, MPF_ERR_STATE = 5, 6, 7
 STATUS_NAMES = {
     0: "MPF_OK", 1: "MPF_ERR_ARG", 2: "MPF_ERR_UNSUPPORTED", 3: "MPF_ERR_HIP", 4: "MPF_ERR_NOMEM",
     5: "MPF_SLOT_OVERFLOW", 6: "MPF_EMPTY_NODE", 7: "MPF_ERR_STATE",
@@ -31,7 +33,7 @@ EXPORTED = [
     "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
     "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_autotune",
     "mpf_set_launch_shape", "mpf_ensemble_create", "mpf_ensemble_step", "mpf_ensemble_destroy",
-    "mpf_error_message", "mpf_destroy",
+    "mpf_ensemble_error_message", "mpf_error_message", "mpf_destroy",
 ]
 
 
@@ -109,6 +111,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_ensemble_create": ([C.POINTER(vp), C.c_int32, C.POINTER(vp)], C.c_int),
         "mpf_ensemble_step": ([vp, f64p, C.c_int32], C.c_int),
         "mpf_ensemble_destroy": ([vp], None),
+        "mpf_ensemble_error_message": ([vp], C.c_char_p),
         "mpf_error_message": ([vp], C.c_char_p),
         "mpf_destroy": ([vp], None),
     }

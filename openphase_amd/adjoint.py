@@ -89,6 +89,7 @@ class _Rollout(torch.autograd.Function):
         cnt, ids, val = states.views(final)
         val, cnt = val.clone(), cnt.clone()
         ctx.states, ctx.checkpoints = states, checkpoints
+        ctx.params = p.copy()
         ctx.nsteps, ctx.dt, ctx.segment = nsteps, dt, segment
         ctx.nphases = eng.cfg.nphases
         ctx.mark_non_differentiable(cnt, ids)
@@ -97,6 +98,18 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g_val, g_cnt, g_ids):
         states, eng = ctx.states, ctx.states.engine
+        # the recomputation and the adjoint steps run with this rollout's tables, whatever the
+        # engine holds now (another rollout may have run since); the caller's tables come back after
+        held = {k: (None if v is None else v.copy()) for k, v in eng.pair_params.items()}
+        p = ctx.params
+        eng.set_pair_params(p[0], p[1], p[2] if eng.cfg.dg_mode is not None else None)
+        try:
+            return _Rollout._backward(ctx, states, eng, g_val)
+        finally:
+            eng.set_pair_params(held["sigma"], held["mu"], held["dg_value"])
+
+    @staticmethod
+    def _backward(ctx, states, eng, g_val):
         npp = ctx.nphases * ctx.nphases
         grad = torch.zeros(3 * npp, dtype=torch.float64, device="cuda")
         lam = torch.zeros(states.slots, states.ncells, dtype=torch.float64, device="cuda")

@@ -109,6 +109,8 @@ struct mpf_ctx {
     uint32_t* d_work_list = nullptr;
     unsigned* d_work_count = nullptr;
     unsigned long long* d_stats = nullptr;
+    unsigned long long* d_obs = nullptr;  // IP.Set maxima: this step's (2 words), the last step's (2)
+    bool stepped = false;                 // a step ran since creation (IP.Set maxima exist)
 
     int xcd_order = 1;               // launch shape, see StepArgs
     int deferred_grid = mpf::kWideGrid;
@@ -206,6 +208,7 @@ mpf::StepArgs step_args(const mpf_ctx* c, double dt, int src)
     a.status = c->d_status;
     a.first_overflow = c->d_first_overflow;
     a.warnings = c->d_warnings;
+    a.obs = c->d_obs;
     a.work_list = c->d_work_list;
     a.work_count = c->d_work_count;
     a.xcd_order = c->xcd_order;
@@ -226,6 +229,7 @@ mpf::GrainArgs grain_args(const mpf_ctx* c)
     g.stage = c->g_stage;
     g.work_count = c->d_work_count;
     g.stats = c->d_stats;
+    g.obs = c->d_obs;
     return g;
 }
 
@@ -475,7 +479,7 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         c->hip(dalloc(&c->d_status, mpf::kStatusWords), "status") ||
         c->hip(dalloc(&c->d_first_overflow, 1), "status") || c->hip(dalloc(&c->d_warnings, 1), "status") ||
         c->hip(dalloc(&c->d_work_list, 4 * N), "work list") || c->hip(dalloc(&c->d_work_count, 4), "work list") ||
-        c->hip(dalloc(&c->d_stats, 2), "statistics"))
+        c->hip(dalloc(&c->d_stats, 2), "statistics") || c->hip(dalloc(&c->d_obs, 4), "statistics"))
         return bail(MPF_ERR_NOMEM, c->err.c_str());
     if (c->dg_avg) {
         const size_t pairs = static_cast<size_t>(c->slots) * (c->slots - 1) / 2;
@@ -504,8 +508,9 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     h2d(e, c->d_warnings, &zero, 1);
     const unsigned zero_u[4] = {0u, 0u, 0u, 0u};
     h2d(e, c->d_work_count, zero_u, 4);
-    const unsigned long long zero_s[2] = {0ull, 0ull};
+    const unsigned long long zero_s[4] = {0ull, 0ull, 0ull, 0ull};
     h2d(e, c->d_stats, zero_s, 2);
+    h2d(e, c->d_obs, zero_s, 4);
     if ((s = c->hip(e, "init copies"))) return bail(s, c->err.c_str());
     *out = c;
     return MPF_OK;
@@ -516,8 +521,8 @@ mpf_status mpf_set_grains(mpf_ctx* c, int32_t ngrains, const int32_t* phase, con
 {
     if (!c) return MPF_ERR_ARG;
     if (ngrains < 1 || ngrains > 65536) return c->fail(MPF_ERR_ARG, "ngrains must be in [1, 65536]");
-    free_grains(c);
-    c->ngrains = ngrains;
+    // every argument is checked and every device array allocated before the current grains are
+    // replaced: a failed call leaves the context as it was
     const size_t G = static_cast<size_t>(ngrains);
     std::vector<int32_t> ph(G, 0);
     std::vector<double> rot(9 * G, 0.0), ref(G), zero(G, 0.0);
@@ -540,28 +545,46 @@ mpf_status mpf_set_grains(mpf_ctx* c, int32_t ngrains, const int32_t* phase, con
     rv = rv * c->nucleus_volume_factor;
     std::fill(ref.begin(), ref.end(), rv);
 
-    if (c->hip(dalloc(&c->g_phase, G), "grains") || c->hip(dalloc(&c->g_vr, G), "grains") ||
-        c->hip(dalloc(&c->g_rot, 9 * G), "grains") || c->hip(dalloc(&c->g_solid, G), "grains") ||
-        c->hip(dalloc(&c->g_volume, G), "grains") || c->hip(dalloc(&c->g_maxvolume, G), "grains") ||
-        c->hip(dalloc(&c->g_refvolume, G), "grains") || c->hip(dalloc(&c->g_exist, G), "grains") ||
-        c->hip(dalloc(&c->g_stage, G), "grains") || c->hip(dalloc(&c->g_acc, G * mpf::kLimbs), "grains"))
-        return MPF_ERR_NOMEM;
+    mpf_ctx n;  // the new arrays; moved into c only when complete
+    auto drop = [&](mpf_status s) {
+        free_grains(&n);
+        return s;
+    };
+    if (c->hip(dalloc(&n.g_phase, G), "grains") || c->hip(dalloc(&n.g_vr, G), "grains") ||
+        c->hip(dalloc(&n.g_rot, 9 * G), "grains") || c->hip(dalloc(&n.g_solid, G), "grains") ||
+        c->hip(dalloc(&n.g_volume, G), "grains") || c->hip(dalloc(&n.g_maxvolume, G), "grains") ||
+        c->hip(dalloc(&n.g_refvolume, G), "grains") || c->hip(dalloc(&n.g_exist, G), "grains") ||
+        c->hip(dalloc(&n.g_stage, G), "grains") || c->hip(dalloc(&n.g_acc, G * mpf::kLimbs), "grains"))
+        return drop(MPF_ERR_NOMEM);
     std::vector<unsigned long long> acc(G * mpf::kLimbs, 0ull);
     hipError_t e = hipSuccess;
-    h2d(e, c->g_phase, ph.data(), G);
-    h2d(e, c->g_vr, zero.data(), G);
-    h2d(e, c->g_rot, rot.data(), 9 * G);
-    h2d(e, c->g_solid, sol.data(), G);
-    h2d(e, c->g_volume, zero.data(), G);
-    h2d(e, c->g_maxvolume, zero.data(), G);
-    h2d(e, c->g_refvolume, ref.data(), G);
-    h2d(e, c->g_exist, ex.data(), G);
-    h2d(e, c->g_stage, stage.data(), G);
-    h2d(e, c->g_acc, acc.data(), G * mpf::kLimbs);
-    mpf_status s = c->hip(e, "grain upload");
-    if (s) return s;
+    h2d(e, n.g_phase, ph.data(), G);
+    h2d(e, n.g_vr, zero.data(), G);
+    h2d(e, n.g_rot, rot.data(), 9 * G);
+    h2d(e, n.g_solid, sol.data(), G);
+    h2d(e, n.g_volume, zero.data(), G);
+    h2d(e, n.g_maxvolume, zero.data(), G);
+    h2d(e, n.g_refvolume, ref.data(), G);
+    h2d(e, n.g_exist, ex.data(), G);
+    h2d(e, n.g_stage, stage.data(), G);
+    h2d(e, n.g_acc, acc.data(), G * mpf::kLimbs);
+    if (mpf_status s = c->hip(e, "grain upload")) return drop(s);
+    // work queued on the stream may still read the old arrays
+    if (mpf_status s = c->hip(hipStreamSynchronize(c->stream), "stream synchronize")) return drop(s);
+    free_grains(c);
+    c->g_phase = n.g_phase; c->g_vr = n.g_vr; c->g_rot = n.g_rot; c->g_solid = n.g_solid;
+    c->g_volume = n.g_volume; c->g_maxvolume = n.g_maxvolume; c->g_refvolume = n.g_refvolume;
+    c->g_exist = n.g_exist; c->g_stage = n.g_stage; c->g_acc = n.g_acc;
+    n.g_phase = nullptr; n.g_vr = nullptr; n.g_rot = nullptr; n.g_solid = nullptr;
+    n.g_volume = nullptr; n.g_maxvolume = nullptr; n.g_refvolume = nullptr;
+    n.g_exist = nullptr; n.g_stage = nullptr; n.g_acc = nullptr;
+    // an uploaded state was checked against the previous grain count: with fewer grains its
+    // indices may be out of range, so it has to be uploaded again
+    if (ngrains < c->ngrains) c->have_state = false;
+    c->ngrains = ngrains;
     c->have_grains = true;
     c->initialized = false;
+    c->args_ok = false;
     return MPF_OK;
 }
 
@@ -649,6 +672,7 @@ mpf_status mpf_step_async(mpf_ctx* c, double dt, int32_t nsteps)
         if (!st) st = record(3 * s + 2);
         if (st) return st;
         c->cur = 1 - c->cur;
+        c->stepped = true;
     }
     if (timing) {
         if (mpf_status st = c->hip(hipEventSynchronize(c->events[3 * (nsteps - 1) + 2]), "hipEventSynchronize")) return st;
@@ -779,10 +803,15 @@ mpf_status mpf_get_grain_records(mpf_ctx* c, double* maxvolume, double* refvolum
 mpf_status mpf_max_time_step(mpf_ctx* c, double* dt_max)
 {
     if (!c || !dt_max) return MPF_ERR_ARG;
+    // maxEnergies / maxMobilities as the last step's IP.Set left them; 0 before any step
     double me = 0.0, mm = 0.0;
-    for (size_t i = 0; i < c->maxe.size(); ++i) {
-        me = std::max(me, c->maxe[i]);
-        mm = std::max(mm, c->mu[i]);
+    if (c->stepped) {
+        unsigned long long w[2] = {0ull, 0ull};
+        mpf_status s = c->hip(hipMemcpyAsync(w, c->d_obs + 2, sizeof(w), hipMemcpyDeviceToHost, c->stream), "maxima");
+        if (!s) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+        if (s) return s;
+        std::memcpy(&me, &w[0], sizeof(double));
+        std::memcpy(&mm, &w[1], sizeof(double));
     }
     double dt = DBL_MAX;
     if (me > DBL_EPSILON && mm > 0.0) {
@@ -894,7 +923,7 @@ void mpf_destroy(mpf_ctx* c)
     dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
-    dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats);
+    dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats); dfree(c->d_obs);
     dfree(c->d_avg_tmp); dfree(c->d_avg); dfree(c->d_avg_off); dfree(c->d_avg_off_w);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1064,13 +1093,25 @@ mpf_status mpf_autotune(mpf_ctx* c, double dt, int32_t steps, int32_t* xcd_order
     void* saved = nullptr;
     if (c->hip(hipMalloc(&saved, static_cast<size_t>(nbytes)), "autotune checkpoint")) return MPF_ERR_NOMEM;
     const int cur0 = c->cur;
-    mpf_status st = mpf_save_state(c, saved);
+    const int x0 = c->xcd_order, g0 = c->deferred_grid;
+    // the trials add their steps to the statistics and counters; those are put back afterwards
+    int32_t status0[mpf::kStatusWords];
+    unsigned long long first0 = 0, warn0 = 0, stats0[2] = {0, 0}, obs0[4] = {0, 0, 0, 0};
+    const bool stepped0 = c->stepped;
+    mpf_status st = c->hip(hipMemcpyAsync(status0, c->d_status, sizeof(status0), hipMemcpyDeviceToHost, c->stream), "autotune snapshot");
+    if (!st) st = c->hip(hipMemcpyAsync(&first0, c->d_first_overflow, sizeof(first0), hipMemcpyDeviceToHost, c->stream), "autotune snapshot");
+    if (!st) st = c->hip(hipMemcpyAsync(&warn0, c->d_warnings, sizeof(warn0), hipMemcpyDeviceToHost, c->stream), "autotune snapshot");
+    if (!st) st = c->hip(hipMemcpyAsync(stats0, c->d_stats, sizeof(stats0), hipMemcpyDeviceToHost, c->stream), "autotune snapshot");
+    if (!st) st = c->hip(hipMemcpyAsync(obs0, c->d_obs, sizeof(obs0), hipMemcpyDeviceToHost, c->stream), "autotune snapshot");
+    if (!st) st = c->hip(hipStreamSynchronize(c->stream), "autotune snapshot");
+    if (!st) st = mpf_save_state(c, saved);
+    const bool have_saved = st == MPF_OK;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!st) st = c->hip(hipEventCreate(&e0), "hipEventCreate");
     if (!st) st = c->hip(hipEventCreate(&e1), "hipEventCreate");
     const int grids[4] = {256, 1024, mpf::kWideGrid, 4 * mpf::kWideGrid};
     float best = 1e30f;
-    int best_x = c->xcd_order, best_g = c->deferred_grid;
+    int best_x = x0, best_g = g0;
     for (int x = 0; x < 2 && !st; ++x)
         for (int gi = 0; gi < 4 && !st; ++gi) {
             c->xcd_order = x;
@@ -1087,10 +1128,23 @@ mpf_status mpf_autotune(mpf_ctx* c, double dt, int32_t steps, int32_t* xcd_order
             if (!st) st = read_status(c);
             if (!st && ms < best) { best = ms; best_x = x; best_g = grids[gi]; }
         }
+    if (st) { best_x = x0; best_g = g0; }  // a failed trial keeps the previous shape
     c->xcd_order = best_x;
     c->deferred_grid = best_g;
     c->cur = cur0;
-    if (!st) st = mpf_restore_state(c, saved);
+    // the state and the counters are restored whether or not a trial failed
+    if (have_saved) {
+        (void)hipStreamSynchronize(c->stream);
+        mpf_status rs = mpf_restore_state(c, saved);
+        if (!rs) rs = c->hip(hipMemcpyAsync(c->d_status, status0, sizeof(status0), hipMemcpyHostToDevice, c->stream), "autotune restore");
+        if (!rs) rs = c->hip(hipMemcpyAsync(c->d_first_overflow, &first0, sizeof(first0), hipMemcpyHostToDevice, c->stream), "autotune restore");
+        if (!rs) rs = c->hip(hipMemcpyAsync(c->d_warnings, &warn0, sizeof(warn0), hipMemcpyHostToDevice, c->stream), "autotune restore");
+        if (!rs) rs = c->hip(hipMemcpyAsync(c->d_stats, stats0, sizeof(stats0), hipMemcpyHostToDevice, c->stream), "autotune restore");
+        if (!rs) rs = c->hip(hipMemcpyAsync(c->d_obs, obs0, sizeof(obs0), hipMemcpyHostToDevice, c->stream), "autotune restore");
+        c->stepped = stepped0;
+        if (!rs) rs = c->hip(hipStreamSynchronize(c->stream), "autotune restore");
+        if (!st) st = rs;
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(saved);
@@ -1279,7 +1333,14 @@ mpf_status mpf_ensemble_step(mpf_ensemble* e, const double* dt, int32_t nsteps)
     if (!st && (nsteps & 1))
         for (size_t i = 0; i < n && !st; ++i) st = enqueue_step(e->members[i], dt[i], e->members[i]->stream);
     for (size_t i = 0; i < n && !st; ++i) st = read_status(e->members[i]);
+    if (!st && nsteps > 0)
+        for (auto* m : e->members) m->stepped = true;
     return st;
+}
+
+const char* mpf_ensemble_error_message(mpf_ensemble* e)
+{
+    return e ? e->err.c_str() : "null ensemble";
 }
 
 void mpf_ensemble_destroy(mpf_ensemble* e)

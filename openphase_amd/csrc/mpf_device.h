@@ -61,6 +61,9 @@ struct StepArgs {
     int32_t* status;             // kStatusWords
     unsigned long long* first_overflow;
     unsigned long long* warnings;
+    // largest interface energy and mobility of the pairs this step's IP.Set evaluates (FP64 bit
+    // patterns of non-negative values, atomicMax): InterfaceProperties maxEnergies/maxMobilities
+    unsigned long long* obs;
     uint32_t* work_list;         // deferred cells: four lists of ncells each (mid kernel levels 0-2; wide kernel)
     unsigned* work_count;        // [4] lengths of the four lists
     // launch shape (chosen by mpf_autotune): XCD-contiguous block order of the step kernel, and
@@ -80,6 +83,7 @@ struct GrainArgs {
     int32_t* stage;              // 0 seed, 1 nucleus, 2 stable
     unsigned* work_count;        // [4], reset here for the next step
     unsigned long long* stats;   // [2] cumulative lengths of work lists 0 and 1
+    unsigned long long* obs;     // [4] this step's maxima (StepArgs::obs, cleared here) and the last step's
 };
 
 // Adjoint of one step (mpf_adjoint.hip). `s` describes the state S_t the step starts from.

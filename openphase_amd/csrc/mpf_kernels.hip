@@ -94,6 +94,10 @@ __device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, 
         // is (sigma - sigma) * (...) = 0, so d = sigma*(X_a - X_b), X = L + P*phi
         const TC s0 = static_cast<TC>(A.sigma0);
         const TC mf = static_cast<TC>(A.mu0) * static_cast<TC>(norm_1);
+        if (ne > 1) {
+            L.emax = static_cast<double>(s0);
+            L.mmax = A.mu0;
+        }
         TC X[C];
 #pragma unroll U
         for (int t = 0; t < C; ++t) X[t] = L.lap[t] + P * L.ev(t);
@@ -183,6 +187,8 @@ __device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, 
                 }
             }
             const TC mob = static_cast<TC>(A.mu[ph[a] * NP + ph[b]]);
+            L.emax = smax(L.emax, static_cast<double>(sigma_of(a, b)));
+            L.mmax = smax(L.mmax, static_cast<double>(mob));
             d *= mob * static_cast<TC>(norm_1) * scale;
             // constant driving force on interface cells (flag 2 = more than one own entry)
             if (DG && n0 > 1 && A.dg_mode[ph[a] * NP + ph[b]] == 1) {
@@ -579,6 +585,8 @@ __device__ __forceinline__ void step_block(const StepArgs& A, int64_t base, uint
     CellList<K, TC, GRAD> L;
 #pragma unroll
     for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
+    L.emax = 0.0;
+    L.mmax = 0.0;
     if (tid < nwide) {
         const int local = s_list[tid];
         const int64_t cw = base + local;
@@ -612,6 +620,7 @@ __device__ __forceinline__ void step_block(const StepArgs& A, int64_t base, uint
         }
         if (empty) atomicOr(&A.status[kStatusEmpty], 1);
     }
+    note_maxima(A.obs, L.emax, L.mmax);
     // whole-list walk for FP32 states, per-slot walk for FP64 ones (whose extra live registers
     // make the whole-list loop spill): 3.14 vs 3.60 ms and 3.63 vs 3.86 ms per 256^3 step
     if constexpr (sizeof(TV) == 4) {
@@ -670,6 +679,8 @@ __device__ __forceinline__ void mid_level(const StepArgs& A, unsigned lblock, un
         bool empty = false, bad_volume = false;
 #pragma unroll
         for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
+        L.emax = 0.0;
+        L.mmax = 0.0;
         if (valid) {
             const int64_t c = list[i];
             const int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
@@ -710,6 +721,7 @@ __device__ __forceinline__ void mid_level(const StepArgs& A, unsigned lblock, un
             }
             if (empty) atomicOr(&A.status[kStatusEmpty], 1);
         }
+        note_maxima(A.obs, L.emax, L.mmax);
 #pragma unroll
         for (int t = 0; t < K; ++t) wave_volume_add<true>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
         if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
@@ -754,6 +766,8 @@ __global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs* __rest
         CellList<K, TC, GRAD> L;
         bool empty = false, bad_volume = false;
         for (int t = 0; t < K; ++t) { L.keep[t] = false; L.eid[t] = 0u; L.phi[t] = 0.0; }
+        L.emax = 0.0;
+        L.mmax = 0.0;
         if (valid) {
             const int64_t c = A.work_list[3 * A.ncells + i];
             const int r = gather_cell<K, TV, TC, GRAD>(A, c, L);
@@ -761,6 +775,7 @@ __global__ __launch_bounds__(kWideBlock) void wide_kernel(const StepArgs* __rest
             if (r == kCellDeferred || !store_cell<K, TV, TC, GRAD>(A, c, L)) report_overflow<K, TC, GRAD>(A, c, L);
             if (empty) atomicOr(&A.status[kStatusEmpty], 1);
         }
+        note_maxima(A.obs, L.emax, L.mmax);
         for (int t = 0; t < K; ++t) wave_volume_add<true>(L.keep[t], L.eid[t], L.phi[t], A.vol_acc, bad_volume);
         if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
     }
@@ -870,6 +885,11 @@ __global__ void grains_kernel(const GrainArgs* __restrict__ batch, const uint32_
         G.work_count[1] = 0u;
         G.work_count[2] = 0u;
         G.work_count[3] = 0u;
+        // this step's IP.Set maxima become the reported ones; the next step starts from 0
+        G.obs[2] = G.obs[0];
+        G.obs[3] = G.obs[1];
+        G.obs[0] = 0ull;
+        G.obs[1] = 0ull;
     }
     if (g >= G.ngrains) return;
     unsigned long long l[kLimbs];

@@ -138,6 +138,25 @@ __device__ __forceinline__ void wave_volume_add(bool act, uint32_t g, double v,
     }
 }
 
+// InterfaceProperties::SetSR's maxima (src/InterfaceProperties.cpp:748-756): the largest energy and
+// mobility over the wavefront's cells, merged into the step's words with atomicMax on the bit
+// patterns (non-negative doubles order like their bits). A word already at least as large (a
+// possibly stale read) needs no atomic. Must be reached by whole wavefronts.
+__device__ __forceinline__ void note_maxima(unsigned long long* obs, double emax, double mmax)
+{
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) {
+        emax = fmax(emax, __shfl_xor(emax, k));
+        mmax = fmax(mmax, __shfl_xor(mmax, k));
+    }
+    if ((threadIdx.x & (warpSize - 1)) == 0) {
+        const unsigned long long e = static_cast<unsigned long long>(__double_as_longlong(emax));
+        const unsigned long long m = static_cast<unsigned long long>(__double_as_longlong(mmax));
+        if (e > __atomic_load_n(&obs[0], __ATOMIC_RELAXED)) atomicMax(&obs[0], e);
+        if (m > __atomic_load_n(&obs[1], __ATOMIC_RELAXED)) atomicMax(&obs[1], m);
+    }
+}
+
 // InterfaceOrientation (src/InterfaceProperties.cpp:39-45) and the cubic energy of
 // InterfaceEnergyModel::Calculate (include/InterfaceProperties/InterfaceEnergyModel.h:325-345).
 template <typename TC>
@@ -189,6 +208,7 @@ template <int K, typename TC, bool GRAD> struct CellList {
     bool keep[K];
     int n0;  // own entries (flag 2 when > 1)
     int ne;  // merged entries
+    double emax, mmax;  // largest interface energy / mobility of the cell's pairs (IP.Set maxima)
 };
 
 // Neighbour cells of one cell as 32-bit offsets: nine row starts (ii, jj) plus the z coordinate.
@@ -276,6 +296,8 @@ __device__ __forceinline__ int load_own(const StepArgs& A, int64_t cc, uint8_t c
     const int imp0 = ListCode<TV>::implicit(code);
     L.n0 = n0;
     L.ne = n0;
+    L.emax = 0.0;
+    L.mmax = 0.0;
 #pragma unroll U
     for (int t = 0; t < K; ++t) {
         L.eid[t] = 0u; L.phi[t] = 0.0; L.lap[t] = TC(0); L.keep[t] = false;

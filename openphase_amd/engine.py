@@ -6,6 +6,7 @@ benchmark and the tests. All numerics run in libmpf_hip.so on the GPU.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -118,10 +119,21 @@ class Engine:
             raise MpfError(st, "mpf_create")
         self.h = h
         self.ngrains = 0
+        # phase-pair tables the context currently holds (set_pair_params replaces them)
+        np2 = cfg.nphases * cfg.nphases
+        self.pair_params = {
+            "sigma": np.asarray(cfg.sigma, np.float64).reshape(np2).copy(),
+            "mu": np.asarray(cfg.mu, np.float64).reshape(np2).copy(),
+            "dg_value": None if cfg.dg_value is None else np.asarray(cfg.dg_value, np.float64).reshape(np2).copy(),
+        }
+        self._ensembles = weakref.WeakSet()  # ensembles that hold this context
 
     # -- lifecycle --------------------------------------------------------------------------
     def close(self) -> None:
+        """Destroys the context; refused while an open Ensemble holds it (close that first)."""
         if getattr(self, "h", None):
+            if any(e.h for e in getattr(self, "_ensembles", ())):
+                raise RuntimeError("Engine.close: an open Ensemble holds this context; close the Ensemble first")
             self.lib.mpf_destroy(self.h)
             self.h = None
 
@@ -193,6 +205,13 @@ class Engine:
                                                    ptr(ex, C.c_uint8), ptr(st, C.c_int32)), "mpf_get_grain_records")
         return mv, rv, ex, st
 
+    def max_time_step(self) -> float:
+        """InterfaceProperties::ReportMaximumTimeStep from the maxima the last step's IP.Set saw
+        (DBL_MAX before the first step)."""
+        v = C.c_double()
+        self._check(self.lib.mpf_max_time_step(self.h, C.byref(v)), "mpf_max_time_step")
+        return v.value
+
     def limiter_warnings(self) -> int:
         w = C.c_int64()
         self._check(self.lib.mpf_limiter_warnings(self.h, C.byref(w)), "mpf_limiter_warnings")
@@ -243,6 +262,9 @@ class Engine:
         self._keep = (s, m, g)
         self._check(self.lib.mpf_set_pair_params(self.h, ptr(s, C.c_double), ptr(m, C.c_double), ptr(g, C.c_double)),
                     "mpf_set_pair_params")
+        for key, a in (("sigma", s), ("mu", m), ("dg_value", g)):
+            if a is not None:
+                self.pair_params[key] = a.copy()
 
     def state_bytes(self) -> int:
         n = C.c_int64()
@@ -286,12 +308,15 @@ class Ensemble:
         if st != _capi.MPF_OK:
             raise MpfError(st, "mpf_ensemble_create", "")
         self.h = h
+        for e in self.engines:
+            e._ensembles.add(self)
 
     def step(self, dt, nsteps: int) -> None:
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (len(self.engines),)))
         st = self.lib.mpf_ensemble_step(self.h, ptr(d, C.c_double), int(nsteps))
         if st != _capi.MPF_OK:
-            msgs = [e.lib.mpf_error_message(e.h) for e in self.engines]
+            own = self.lib.mpf_ensemble_error_message(self.h)
+            msgs = [own] + [e.lib.mpf_error_message(e.h) for e in self.engines]
             raise MpfError(st, "mpf_ensemble_step", "; ".join(m.decode() for m in msgs if m))
 
     def close(self) -> None:

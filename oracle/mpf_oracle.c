@@ -397,6 +397,128 @@ static double max_energy_of(const orc_ctx* c, int pa, int pb)
     return c->sigma[k];
 }
 
+/* PhaseField::NormalizeIncrementsSR (src/PhaseField.cpp:1351-1567) on one node: the single-pair
+ * branch (:1368-1404), zeroing of pairs that push a field at 0 below or at 1 above (:1407-1431),
+ * removal of zero pairs, the iterative limiter with its 25-pass cap (:1451-1519), removal of
+ * increments below DBL_EPSILON (:1521-1533) and the plausibility check, whose warnings are counted
+ * (:1536-1561). */
+static int normalize_pairs(const orc_node* nd, orc_pair* dot, int* nd_dot, double dt, int64_t* warnings)
+{
+    const int n = nd->n;
+    if ((*nd_dot) == 1) {
+        uint32_t ia = dot[0].a;
+        double va = (dot[0].v1 + dot[0].v2) * dt;
+        double old = node_get_value(nd, ia);
+        if ((old == 0.0 && va < 0.0) || (old == 1.0 && va > 0.0)) {
+            (*nd_dot) = 0;
+        } else {
+            double nv = old + va;
+            double norm = 1.0;
+            if (nv < 0.0) norm *= -old / va;
+            else if (nv > 1.0) norm *= (1.0 - old) / va;
+            if (norm > DBL_EPSILON) {
+                dot[0].v1 *= norm;
+                dot[0].v2 *= norm;
+            } else {
+                (*nd_dot) = 0;
+            }
+        }
+    } else if ((*nd_dot) > 1) {
+        for (int a = 0; a < n; ++a) {
+            double dpsi = 0.0;
+            for (int b = 0; b < n; ++b) {
+                if (a == b) continue;
+                dpsi += ab_get_asym1(dot, (*nd_dot), nd->e[a].idx, nd->e[b].idx);
+                dpsi += ab_get_asym2(dot, (*nd_dot), nd->e[a].idx, nd->e[b].idx);
+            }
+            if ((nd->e[a].val == 0.0 && dpsi < 0.0) || (nd->e[a].val == 1.0 && dpsi > 0.0)) {
+                for (int b = 0; b < n; ++b) {
+                    if (a == b) continue;
+                    if (ab_set_sym_pair(dot, &(*nd_dot), nd->e[a].idx, nd->e[b].idx, 0.0, 0.0)) {
+                        return ORC_ERR_CAPACITY;
+                    }
+                }
+            }
+        }
+        int w = 0;
+        for (int i = 0; i < (*nd_dot); ++i)
+            if (!(dot[i].v1 == 0.0 && dot[i].v2 == 0.0)) dot[w++] = dot[i];
+        (*nd_dot) = w;
+
+        if ((*nd_dot)) {
+            /* per-index positive/negative sums and limits keyed by phase-field index */
+            double pos[ORC_CAP], neg[ORC_CAP], lim1[ORC_CAP], lim2[ORC_CAP];
+            int iter = 0;
+            int needed = 1;
+            while (needed) {
+                ++iter;
+                needed = 0;
+                for (int a = 0; a < n; ++a) { pos[a] = 0.0; neg[a] = 0.0; }
+                for (int i = 0; i < (*nd_dot); ++i) {
+                    int sa = -1, sb = -1;
+                    for (int a = 0; a < n; ++a) {
+                        if (nd->e[a].idx == dot[i].a) sa = a;
+                        if (nd->e[a].idx == dot[i].b) sb = a;
+                    }
+                    if (dot[i].v1 < 0.0) {
+                        neg[sa] += dot[i].v1;
+                        pos[sb] += -dot[i].v1;
+                    }
+                    if (dot[i].v1 > 0.0) {
+                        pos[sa] += dot[i].v1;
+                        neg[sb] += -dot[i].v1;
+                    }
+                }
+                for (int a = 0; a < n; ++a) {
+                    double phi = nd->e[a].val;
+                    double npf = phi + (pos[a] + neg[a]) * dt;
+                    lim2[a] = 1.0;
+                    lim1[a] = 1.0;
+                    if (npf < 0.0) lim2[a] = std_min(lim2[a], -(phi + pos[a] * dt) / (neg[a] * dt));
+                    if (npf > 1.0) lim1[a] = std_min(lim1[a], (1.0 - (phi + neg[a] * dt)) / (pos[a] * dt));
+                }
+                for (int i = 0; i < (*nd_dot); ++i) {
+                    int sa = -1, sb = -1;
+                    for (int a = 0; a < n; ++a) {
+                        if (nd->e[a].idx == dot[i].a) sa = a;
+                        if (nd->e[a].idx == dot[i].b) sb = a;
+                    }
+                    if (dot[i].v1 < 0.0) {
+                        double t = std_min(lim2[sa], lim1[sb]);
+                        dot[i].v1 *= t;
+                        if (t < 1.0) needed = 1;
+                    }
+                    if (dot[i].v1 > 0.0) {
+                        double t = std_min(lim1[sa], lim2[sb]);
+                        dot[i].v1 *= t;
+                        if (t < 1.0) needed = 1;
+                    }
+                }
+                if (iter > 24) needed = 0;
+            }
+            w = 0;
+            for (int i = 0; i < (*nd_dot); ++i)
+                if (!(fabs(dot[i].v1 * dt) < DBL_EPSILON)) dot[w++] = dot[i];
+            (*nd_dot) = w;
+
+            /* plausibility check: counts the warnings the reference prints */
+            double tv[ORC_CAP];
+            for (int a = 0; a < n; ++a) tv[a] = nd->e[a].val;
+            for (int i = 0; i < (*nd_dot); ++i) {
+                if (dot[i].v1 == 0.0) continue;
+                for (int a = 0; a < n; ++a) {
+                    if (nd->e[a].idx == dot[i].a) tv[a] += dot[i].v1 * dt;
+                    if (nd->e[a].idx == dot[i].b) tv[a] += -dot[i].v1 * dt;
+                }
+            }
+            for (int a = 0; a < n; ++a)
+                if (tv[a] < -FLT_EPSILON || tv[a] > 1.0 + FLT_EPSILON) ++*warnings;
+        }
+    }
+
+    return 0;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* One node: IP.Set -> DO increments (-> driving force) -> NormalizeIncrements -> MergeIncrements */
 
@@ -523,116 +645,9 @@ static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
     }
 
     /* PhaseField::NormalizeIncrementsSR (src/PhaseField.cpp:1351-1567) */
-    if (nd_dot == 1) {
-        uint32_t ia = dot[0].a;
-        double va = (dot[0].v1 + dot[0].v2) * dt;
-        double old = node_get_value(nd, ia);
-        if ((old == 0.0 && va < 0.0) || (old == 1.0 && va > 0.0)) {
-            nd_dot = 0;
-        } else {
-            double nv = old + va;
-            double norm = 1.0;
-            if (nv < 0.0) norm *= -old / va;
-            else if (nv > 1.0) norm *= (1.0 - old) / va;
-            if (norm > DBL_EPSILON) {
-                dot[0].v1 *= norm;
-                dot[0].v2 *= norm;
-            } else {
-                nd_dot = 0;
-            }
-        }
-    } else if (nd_dot > 1) {
-        for (int a = 0; a < n; ++a) {
-            double dpsi = 0.0;
-            for (int b = 0; b < n; ++b) {
-                if (a == b) continue;
-                dpsi += ab_get_asym1(dot, nd_dot, nd->e[a].idx, nd->e[b].idx);
-                dpsi += ab_get_asym2(dot, nd_dot, nd->e[a].idx, nd->e[b].idx);
-            }
-            if ((nd->e[a].val == 0.0 && dpsi < 0.0) || (nd->e[a].val == 1.0 && dpsi > 0.0)) {
-                for (int b = 0; b < n; ++b) {
-                    if (a == b) continue;
-                    if (ab_set_sym_pair(dot, &nd_dot, nd->e[a].idx, nd->e[b].idx, 0.0, 0.0)) {
-                        res->err = ORC_ERR_CAPACITY;
-                        return;
-                    }
-                }
-            }
-        }
-        int w = 0;
-        for (int i = 0; i < nd_dot; ++i)
-            if (!(dot[i].v1 == 0.0 && dot[i].v2 == 0.0)) dot[w++] = dot[i];
-        nd_dot = w;
-
-        if (nd_dot) {
-            /* per-index positive/negative sums and limits keyed by phase-field index */
-            double pos[ORC_CAP], neg[ORC_CAP], lim1[ORC_CAP], lim2[ORC_CAP];
-            int iter = 0;
-            int needed = 1;
-            while (needed) {
-                ++iter;
-                needed = 0;
-                for (int a = 0; a < n; ++a) { pos[a] = 0.0; neg[a] = 0.0; }
-                for (int i = 0; i < nd_dot; ++i) {
-                    int sa = -1, sb = -1;
-                    for (int a = 0; a < n; ++a) {
-                        if (nd->e[a].idx == dot[i].a) sa = a;
-                        if (nd->e[a].idx == dot[i].b) sb = a;
-                    }
-                    if (dot[i].v1 < 0.0) {
-                        neg[sa] += dot[i].v1;
-                        pos[sb] += -dot[i].v1;
-                    }
-                    if (dot[i].v1 > 0.0) {
-                        pos[sa] += dot[i].v1;
-                        neg[sb] += -dot[i].v1;
-                    }
-                }
-                for (int a = 0; a < n; ++a) {
-                    double phi = nd->e[a].val;
-                    double npf = phi + (pos[a] + neg[a]) * dt;
-                    lim2[a] = 1.0;
-                    lim1[a] = 1.0;
-                    if (npf < 0.0) lim2[a] = std_min(lim2[a], -(phi + pos[a] * dt) / (neg[a] * dt));
-                    if (npf > 1.0) lim1[a] = std_min(lim1[a], (1.0 - (phi + neg[a] * dt)) / (pos[a] * dt));
-                }
-                for (int i = 0; i < nd_dot; ++i) {
-                    int sa = -1, sb = -1;
-                    for (int a = 0; a < n; ++a) {
-                        if (nd->e[a].idx == dot[i].a) sa = a;
-                        if (nd->e[a].idx == dot[i].b) sb = a;
-                    }
-                    if (dot[i].v1 < 0.0) {
-                        double t = std_min(lim2[sa], lim1[sb]);
-                        dot[i].v1 *= t;
-                        if (t < 1.0) needed = 1;
-                    }
-                    if (dot[i].v1 > 0.0) {
-                        double t = std_min(lim1[sa], lim2[sb]);
-                        dot[i].v1 *= t;
-                        if (t < 1.0) needed = 1;
-                    }
-                }
-                if (iter > 24) needed = 0;
-            }
-            w = 0;
-            for (int i = 0; i < nd_dot; ++i)
-                if (!(fabs(dot[i].v1 * dt) < DBL_EPSILON)) dot[w++] = dot[i];
-            nd_dot = w;
-
-            /* plausibility check: counts the warnings the reference prints */
-            double tv[ORC_CAP];
-            for (int a = 0; a < n; ++a) tv[a] = nd->e[a].val;
-            for (int i = 0; i < nd_dot; ++i) {
-                if (dot[i].v1 == 0.0) continue;
-                for (int a = 0; a < n; ++a) {
-                    if (nd->e[a].idx == dot[i].a) tv[a] += dot[i].v1 * dt;
-                    if (nd->e[a].idx == dot[i].b) tv[a] += -dot[i].v1 * dt;
-                }
-            }
-            for (int a = 0; a < n; ++a)
-                if (tv[a] < -FLT_EPSILON || tv[a] > 1.0 + FLT_EPSILON) res->warnings++;
-        }
+    {
+        int st = normalize_pairs(nd, dot, &nd_dot, dt, &res->warnings);
+        if (st) { res->err = st; return; }
     }
 
     /* PhaseField::MergeIncrementsSR (src/PhaseField.cpp:1268-1299) */
@@ -1029,4 +1044,34 @@ void orc_list_stats(const orc_ctx* c, int32_t* max_merged, int32_t* max_finalize
 {
     if (max_merged) *max_merged = c->max_list;
     if (max_finalized) *max_finalized = c->max_final;
+}
+
+int orc_normalize_pairs(int32_t n, const int32_t* idx, const double* val, int32_t* npairs_io, int32_t* pa, int32_t* pb,
+                        double* v1, double dt, int64_t* warnings)
+{
+    const int npairs = *npairs_io;
+    if (n < 0 || n > ORC_CAP || npairs < 0 || npairs > ORC_PCAP) return ORC_ERR_CAPACITY;
+    orc_node nd;
+    memset(&nd, 0, sizeof(nd));
+    nd.n = n;
+    nd.flag = 2;
+    for (int i = 0; i < n; ++i) {
+        nd.e[i].idx = (uint32_t)idx[i];
+        nd.e[i].val = val[i];
+    }
+    orc_pair dot[ORC_PCAP];
+    int nd_dot = 0;
+    for (int i = 0; i < npairs; ++i)
+        if (ab_add_asym1(dot, &nd_dot, (uint32_t)pa[i], (uint32_t)pb[i], v1[i])) return ORC_ERR_CAPACITY;
+    int64_t w = 0;
+    int st = normalize_pairs(&nd, dot, &nd_dot, dt, &w);
+    if (st) return st;
+    for (int i = 0; i < nd_dot; ++i) {
+        pa[i] = (int32_t)dot[i].a;
+        pb[i] = (int32_t)dot[i].b;
+        v1[i] = dot[i].v1;
+    }
+    *npairs_io = nd_dot;
+    if (warnings) *warnings = w;
+    return ORC_OK;
 }

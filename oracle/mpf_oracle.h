@@ -84,6 +84,13 @@ int orc_grains(const orc_ctx* c, double* volume, double* volume_ratio);
 int orc_grain_records(const orc_ctx* c, double* maxvolume, double* refvolume, uint8_t* exist, int32_t* stage);
 /* Limiter plausibility warnings seen so far (PhaseField.cpp:1536-1561). */
 int64_t orc_limiter_warnings(const orc_ctx* c);
+
+/* PhaseField::NormalizeIncrementsSR on one interface node given directly: its list (n entries,
+ * phase-field index and value, in list order) and its pair increments (npairs in: the pairs added
+ * with NodeAB::add_asym1 in that order; out: the pairs left, in storage order, with their limited
+ * value1). Counts the plausibility warnings the reference would print. */
+int orc_normalize_pairs(int32_t n, const int32_t* idx, const double* val, int32_t* npairs, int32_t* pa, int32_t* pb,
+                        double* v1, double dt, int64_t* warnings);
 /* InterfaceProperties::ReportMaximumTimeStep for the last IP.Set (src/InterfaceProperties.cpp:523-556). */
 double orc_max_time_step(const orc_ctx* c);
 void orc_set_threads(int32_t n);

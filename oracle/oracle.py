@@ -59,6 +59,8 @@ def load():
         lib.orc_max_time_step.argtypes = [vp]
         lib.orc_max_time_step.restype = C.c_double
         lib.orc_set_threads.argtypes = [C.c_int32]
+        lib.orc_normalize_pairs.argtypes = [C.c_int32, i32p, f64p, i32p, i32p, i32p, f64p, C.c_double,
+                                            C.POINTER(C.c_int64)]
         _lib = lib
     return _lib
 
@@ -172,6 +174,10 @@ class Oracle:
     def limiter_warnings(self):
         return int(self.lib.orc_limiter_warnings(self.h))
 
+    def max_time_step(self):
+        """InterfaceProperties::ReportMaximumTimeStep from the maxima the last step's IP.Set saw."""
+        return float(self.lib.orc_max_time_step(self.h))
+
     def close(self):
         if self.h:
             self.lib.orc_destroy(self.h)
@@ -184,11 +190,40 @@ class Oracle:
             pass
 
 
-def write_opi(path, cfg, threads: int = 1) -> None:
+def normalize_pairs(idx, val, pairs, dt):
+    """PhaseField::NormalizeIncrementsSR on one interface node (orc_normalize_pairs): `idx`, `val`
+    its list, `pairs` [(a, b, value)] added in order with NodeAB::add_asym1. Returns (the pairs
+    kept, in storage order, with their limited values; the number of plausibility warnings)."""
+    lib = load()
+    n = len(idx)
+    cap = max(len(pairs), 1)
+    ia = np.zeros(cap, np.int32)
+    ib = np.zeros(cap, np.int32)
+    v = np.zeros(cap, np.float64)
+    for q, (a, b, x) in enumerate(pairs):
+        ia[q], ib[q], v[q] = a, b, x
+    npairs = C.c_int32(len(pairs))
+    w = C.c_int64(0)
+    ii = np.ascontiguousarray(idx, np.int32)
+    vv = np.ascontiguousarray(val, np.float64)
+    st = lib.orc_normalize_pairs(n, _p(ii, C.c_int32), _p(vv, C.c_double), C.byref(npairs), _p(ia, C.c_int32),
+                                 _p(ib, C.c_int32), _p(v, C.c_double), float(dt), C.byref(w))
+    if st:
+        raise RuntimeError(f"orc_normalize_pairs failed: {st}")
+    k = npairs.value
+    return [(int(ia[q]), int(ib[q]), float(v[q])) for q in range(k)], int(w.value)
+
+
+def write_opi(path, cfg, threads: int = 1, dt: float | None = None, nsteps: int | None = None) -> None:
     """Input file for the reference driver with the keys the reference modules read
-    (GridParameters, Settings, InterfaceProperties, PhaseField, BoundaryConditions,
+    (RunTimeControl, GridParameters, Settings, InterfaceProperties, PhaseField, BoundaryConditions,
     UserDrivingForce, DrivingForce)."""
-    lines = ["Generated input", "", "@RunTimeControl", f"$nOMP : {threads}", "",
+    rtc = [f"$nOMP : {threads}"]
+    if dt is not None:
+        rtc.append(f"$dt : {float(dt)!r}")
+    if nsteps is not None:
+        rtc.append(f"$nSteps : {int(nsteps)}")
+    lines = ["Generated input", "", "@RunTimeControl", *rtc, "",
              "@GridParameters", f"$Nx : {cfg.nx}", f"$Ny : {cfg.ny}", f"$Nz : {cfg.nz}",
              f"$dx : {cfg.dx!r}", f"$IWidth : {cfg.iwidth!r}", "", "@Settings"]
     for n in range(cfg.nphases):
@@ -269,6 +304,8 @@ def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dum
     if res.returncode != 0:
         raise RuntimeError(f"ref_driver failed ({res.returncode}): {res.stdout[-2000:]} {res.stderr[-2000:]}")
     timing = json.loads([ln for ln in res.stdout.splitlines() if ln.startswith("{\"cells\"")][-1])
+    # PhaseField::NormalizeIncrementsSR's plausibility warnings (src/PhaseField.cpp:1547-1559)
+    timing["limiter_warnings"] = res.stderr.count("Normalizing of phase field increments failed")
     out = {}
     ng = len(grain_phase)
     for s in dumps:

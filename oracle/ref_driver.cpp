@@ -10,13 +10,20 @@
 // zero-valued entries that CalculateDerivativesSR appends (src/PhaseField.cpp:566-615).
 //
 // usage: ref_driver <input.opi> <state.dat> <grains.bin> <nsteps> <dt> <outprefix> <dumpsteps,...>
-//        [--driving-force] [--threads N] [--write-files]
+//        [--driving-force] [--threads N] [--write-files] [--normalize-only <pairs.bin>]
+// --normalize-only: after FinalizeInitialization, adds the pair increments of pairs.bin to
+// Phi.FieldsDot with NodeAB::add_asym1 (uint32 ncells; per cell uint32 linear index (x outer,
+// z inner), uint32 npairs, npairs x {uint32 a; uint32 b; double value}), runs only
+// Phi.NormalizeIncrements(BC, dt) and writes <outprefix>_pairs.bin: per listed cell uint32 npairs,
+// then npairs x {uint32 indexA; uint32 indexB; double value1} as FieldsDot holds them.
 // --write-files: at the dump steps also the reference's own writers: PhaseField::Write (full lists,
 // zero-valued entries included) to <outprefix>_<step>_full.dat, GrainsProperties::Write to
 // <outprefix>_<step>.grains and PhaseField::WriteVTK into the settings' VTK directory.
 // grains.bin: uint32 n; n x { uint32 phase; double q[4] }   (q = s, x, y, z)
 // Writes <outprefix>_<step>.dat and <outprefix>_<step>.vol (n doubles Volume, n doubles VolumeRatio)
-// and prints one JSON line with per-call wall times.
+// and prints one JSON line with per-call wall times, InterfaceProperties::ReportMaximumTimeStep
+// before the first IP.Set ("max_dt_init") and after the IP.Set of every step ("max_dt", one per
+// step). The limiter's plausibility warnings go to stderr as the reference writes them.
 
 #include "Settings.h"
 #include "RunTimeControl.h"
@@ -88,6 +95,7 @@ int main(int argc, char* argv[])
         while (std::getline(ss, tok, ',')) if (!tok.empty()) dumps.insert(std::atoi(tok.c_str()));
     }
     bool with_dg = false, write_files = false;
+    std::string normalize_only;
     int threads = 1;
     for (int a = 8; a < argc; ++a)
     {
@@ -95,6 +103,7 @@ int main(int argc, char* argv[])
         if (s == "--driving-force") with_dg = true;
         if (s == "--write-files") write_files = true;
         if (s == "--threads" && a + 1 < argc) threads = std::atoi(argv[++a]);
+        if (s == "--normalize-only" && a + 1 < argc) normalize_only = argv[++a];
     }
     omp_set_num_threads(threads);
 
@@ -128,6 +137,53 @@ int main(int argc, char* argv[])
     Phi.FinalizeInitialization(BC);
     if (dumps.count(0)) dump_state(Phi, prefix, 0);
 
+    if (!normalize_only.empty())
+    {
+        std::ifstream pin(normalize_only, std::ios::binary);
+        uint32_t nc = 0;
+        pin.read(reinterpret_cast<char*>(&nc), sizeof(uint32_t));
+        const long NY = Phi.Grid.Ny, NZ = Phi.Grid.Nz;
+        std::vector<uint32_t> cells(nc);
+        for (uint32_t c = 0; c < nc; ++c)
+        {
+            uint32_t lin = 0, np = 0;
+            pin.read(reinterpret_cast<char*>(&lin), sizeof(uint32_t));
+            pin.read(reinterpret_cast<char*>(&np), sizeof(uint32_t));
+            cells[c] = lin;
+            const long i = lin / (NY * NZ), j = (lin / NZ) % NY, k = lin % NZ;
+            for (uint32_t q = 0; q < np; ++q)
+            {
+                uint32_t a = 0, b = 0;
+                double v = 0.0;
+                pin.read(reinterpret_cast<char*>(&a), sizeof(uint32_t));
+                pin.read(reinterpret_cast<char*>(&b), sizeof(uint32_t));
+                pin.read(reinterpret_cast<char*>(&v), sizeof(double));
+                Phi.FieldsDot(i, j, k).add_asym1(a, b, v);
+            }
+        }
+        Phi.NormalizeIncrements(BC, dt);
+        std::ofstream pout(prefix + "_pairs.bin", std::ios::binary);
+        for (uint32_t lin : cells)
+        {
+            const long i = lin / (NY * NZ), j = (lin / NZ) % NY, k = lin % NZ;
+            const auto& node = Phi.FieldsDot(i, j, k);
+            uint32_t np = static_cast<uint32_t>(node.size());
+            pout.write(reinterpret_cast<const char*>(&np), sizeof(uint32_t));
+            for (auto it = node.cbegin(); it != node.cend(); ++it)
+            {
+                uint32_t a = static_cast<uint32_t>(it->indexA), b = static_cast<uint32_t>(it->indexB);
+                double v = it->value1;
+                pout.write(reinterpret_cast<const char*>(&a), sizeof(uint32_t));
+                pout.write(reinterpret_cast<const char*>(&b), sizeof(uint32_t));
+                pout.write(reinterpret_cast<const char*>(&v), sizeof(double));
+            }
+        }
+        std::printf("{\"cells\": %ld, \"normalized\": %u}\n", (long)Phi.Grid.Nx * Phi.Grid.Ny * Phi.Grid.Nz, nc);
+        return 0;
+    }
+
+    const double max_dt_init = IP.ReportMaximumTimeStep();
+    std::vector<double> max_dt;
     double t_ip = 0, t_do = 0, t_norm = 0, t_merge = 0, t_dg = 0;
     for (int step = 1; step <= nsteps; ++step)
     {
@@ -150,6 +206,7 @@ int main(int argc, char* argv[])
         Phi.MergeIncrements(BC, dt);
         double t5 = omp_get_wtime();
         t_ip += t1 - t0; t_do += t2 - t1; t_dg += t3 - t2; t_norm += t4 - t3; t_merge += t5 - t4;
+        max_dt.push_back(IP.ReportMaximumTimeStep());
         if (dumps.count(step)) dump_state(Phi, prefix, step);
         if (dumps.count(step) && write_files)
         {
@@ -162,9 +219,11 @@ int main(int argc, char* argv[])
     double total = t_ip + t_do + t_dg + t_norm + t_merge;
     std::printf("{\"cells\": %ld, \"steps\": %d, \"threads\": %d, \"IP.Set\": %.6f, \"CalculatePhaseFieldIncrements\": %.6f, "
                 "\"DrivingForce\": %.6f, \"NormalizeIncrements\": %.6f, \"MergeIncrements\": %.6f, \"total_s\": %.6f, "
-                "\"cell_updates_per_s\": %.6e}\n",
+                "\"cell_updates_per_s\": %.6e, \"max_dt_init\": %.17g, \"max_dt\": [",
                 cells, nsteps, threads, t_ip, t_do, t_dg, t_norm, t_merge, total,
-                total > 0 ? double(cells) * nsteps / total : 0.0);
+                total > 0 ? double(cells) * nsteps / total : 0.0, max_dt_init);
+    for (size_t i = 0; i < max_dt.size(); ++i) std::printf("%s%.17g", i ? ", " : "", max_dt[i]);
+    std::printf("]}\n");
     delete dG;
     delete UDF;
     return 0;

@@ -9,7 +9,7 @@ from openphase_amd.engine import StepConfig
 from openphase_amd.microstructure import Lists, quaternion_to_matrix, storage_shape
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-CASES = sorted(p.stem for p in GOLDEN.glob("*.npz") if not p.stem.startswith("io_"))
+CASES = sorted(p.stem for p in GOLDEN.glob("*.npz") if not p.stem.startswith(("io_", "limiter_")))
 IO_CASES = sorted(p.stem for p in GOLDEN.glob("io_*.npz"))
 
 
@@ -40,6 +40,10 @@ class Case:
         shape = storage_shape(self.cfg.nx, self.cfg.ny, self.cfg.nz)
         self.init = Lists(shape, z["init_count"], z["init_index"], z["init_value"])
         self.dumps = [int(s) for s in z["dumps"]]
+        # ReportMaximumTimeStep before the first step and after step s (index s - 1); warnings
+        self.max_dt_init = float(z["max_dt_init"])
+        self.max_dt = z["max_dt"]
+        self.limiter_warnings = int(z["limiter_warnings"])
         self.expect = {
             s: (Lists(shape, z[f"s{s}_count"], z[f"s{s}_index"], z[f"s{s}_value"]), z[f"s{s}_volume"], z[f"s{s}_vr"])
             for s in self.dumps

@@ -137,6 +137,10 @@ def main(only=()):
             dt=dt, grain_phase=phase, quaternions=np.zeros((ng, 4)) if quat is None else quat,
             init_count=lists.count, init_index=lists.index, init_value=lists.value,
             dumps=np.asarray(dumps, np.int32),
+            # InterfaceProperties::ReportMaximumTimeStep before the first IP.Set and after each
+            # step's IP.Set; plausibility warnings of NormalizeIncrements over the run
+            max_dt_init=float(timing["max_dt_init"]), max_dt=np.asarray(timing["max_dt"], np.float64),
+            limiter_warnings=int(timing["limiter_warnings"]),
         )
         for s in dumps:
             st, vol, vr = ref[s]

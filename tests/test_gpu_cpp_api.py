@@ -24,7 +24,7 @@ def test_normal_gg_example_matches_oracle(tmp_path, shape, ngrains, steps):
         pytest.fail("openphase_amd/examples/normal_gg is not built")
     cfg = StepConfig(*shape)
     lists = voronoi_grains(*shape, ngrains, seed=21)
-    O.write_opi(tmp_path / "in.opi", cfg)
+    O.write_opi(tmp_path / "in.opi", cfg, dt=cfg.max_time_step(), nsteps=steps)
     rawio.write_phasefield(str(tmp_path / "in.dat"), lists)
     print(f"normal_gg {shape} grains {ngrains} steps {steps}", flush=True)
     (tmp_path / "files" / "VTK").mkdir(parents=True)
@@ -38,6 +38,9 @@ def test_normal_gg_example_matches_oracle(tmp_path, shape, ngrains, steps):
     orc.finalize_initialization()
     orc.step(cfg.max_time_step(), steps)
     exp, flags = orc.download(with_flags=True)
+    # IP.ReportMaximumTimeStep after the loop: from the maxima of the last IP.Set, as the oracle's
+    reported = float(res.stdout.split("ReportMaximumTimeStep")[1].split()[0])
+    assert reported == orc.max_time_step()
     # PhaseField::Write keeps the zero-valued entries the derivative pass appended
     keep = full.value != 0.0
     cell = np.repeat(np.arange(full.ncells), full.count)
