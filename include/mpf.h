@@ -232,6 +232,21 @@ mpf_status mpf_ensemble_step(mpf_ensemble* ens, const double* dt, int32_t nsteps
 const char* mpf_ensemble_error_message(mpf_ensemble* ens);
 void mpf_ensemble_destroy(mpf_ensemble* ens);
 
+/* One step without any allocation (no reference counterpart; for callers that cannot allocate
+ * device memory: code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Advances the state in the caller's DEVICE buffers count_in / index_in / value_in (the layout of
+ * mpf_device_state: uint8 counts, uint16 indices and FP64 or FP32 values slot-major with the
+ * context's slot capacity, as this library wrote them) by one mpf_step into count_out / index_out /
+ * value_out (same layout, distinct buffers) with the context's grains, tables and launch shape;
+ * the context's own state is not touched, its grain records (Volume, VolumeRatio, ...) become
+ * those of the output state. The step's scratch (deferred-cell work lists, volume accumulators,
+ * status words) is one static device array per GPU, cleared at the start of every call: at most
+ * 2^19 cells per call (MPF_ERR_UNSUPPORTED beyond, and with driving-force averaging), 10 MiB per
+ * GPU held until the process ends. Calls that overlap on one GPU (other threads, other streams)
+ * race on that array. Synchronises on the context's stream and returns the step's status. */
+mpf_status mpf_step_into(mpf_ctx* ctx, double dt, const void* count_in, const void* index_in, const void* value_in,
+                         void* count_out, void* index_out, void* value_out);
+
 /* Human-readable description of the last error on this context (static storage per context). */
 const char* mpf_error_message(mpf_ctx* ctx);
 

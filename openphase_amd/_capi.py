@@ -33,7 +33,7 @@ EXPORTED = [
     "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
     "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_autotune",
     "mpf_set_launch_shape", "mpf_ensemble_create", "mpf_ensemble_step", "mpf_ensemble_destroy",
-    "mpf_ensemble_error_message", "mpf_error_message", "mpf_destroy",
+    "mpf_ensemble_error_message", "mpf_step_into", "mpf_error_message", "mpf_destroy",
 ]
 
 
@@ -112,6 +112,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_ensemble_step": ([vp, f64p, C.c_int32], C.c_int),
         "mpf_ensemble_destroy": ([vp], None),
         "mpf_ensemble_error_message": ([vp], C.c_char_p),
+        "mpf_step_into": ([vp, C.c_double, vp, vp, vp, vp, vp, vp], C.c_int),
         "mpf_error_message": ([vp], C.c_char_p),
         "mpf_destroy": ([vp], None),
     }

@@ -1355,3 +1355,148 @@ void mpf_ensemble_destroy(mpf_ensemble* e)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ step without allocation
+//
+// mpf_step_into keeps the per-step scratch of a step - deferred-cell work lists and their
+// counters, per-grain volume accumulators, status words and the argument blocks the deferred-cell
+// and grain kernels read - in one file-scope device array instead of context allocations, and it
+// reads and writes the caller's state buffers. Each GPU holds its own copy of the array from
+// module load until the process ends; its device address is looked up once per device.
+
+namespace {
+
+constexpr int64_t kScratchCells = int64_t{1} << 19;  // cells per call (2D 724^2, 3D 80^3)
+constexpr int kScratchGrains = 65536;                  // every grain count the library allows
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t{255}; }
+// layout of the scratch array (byte offsets, 256-byte aligned parts)
+constexpr size_t kOffStepArgs = 0;
+constexpr size_t kOffGrainArgs = align256(kOffStepArgs + sizeof(mpf::StepArgs));
+constexpr size_t kOffFirst = align256(kOffGrainArgs + sizeof(mpf::GrainArgs));      // 2 x uint32
+constexpr size_t kOffCount = align256(kOffFirst + 2 * sizeof(uint32_t));            // 4 x uint32
+constexpr size_t kOffStatus = align256(kOffCount + 4 * sizeof(unsigned));           // 4 x int32
+constexpr size_t kOffWords = align256(kOffStatus + mpf::kStatusWords * sizeof(int32_t));  // 8 x u64
+constexpr size_t kOffAcc = align256(kOffWords + 8 * sizeof(unsigned long long));
+constexpr size_t kOffList = align256(kOffAcc + size_t{kScratchGrains} * mpf::kLimbs * sizeof(unsigned long long));
+constexpr size_t kScratchBytes = align256(kOffList + 4 * size_t{kScratchCells} * sizeof(uint32_t));
+
+// 10 MiB + 2.2 KiB per GPU
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device, 0 = not looked up
+
+// Device address of this device's `scratch`: looked up on the first call on a device, with that
+// device current (it is the current device of the calling thread).
+mpf_status scratch_address(mpf_ctx* c, unsigned char** out)
+{
+    int dev = 0;
+    if (mpf_status s = c->hip(hipGetDevice(&dev), "hipGetDevice")) return s;
+    if (dev < 0 || dev >= kMaxDevices) return c->fail(MPF_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    void* p = __atomic_load_n(&g_scratch_ptr[dev], __ATOMIC_ACQUIRE);
+    if (!p) {
+        if (mpf_status s = c->hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return s;
+        if (!p) return c->fail(MPF_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        __atomic_store_n(&g_scratch_ptr[dev], p, __ATOMIC_RELEASE);
+    }
+    *out = static_cast<unsigned char*>(p);
+    return MPF_OK;
+}
+
+// Clears what a step expects at zero (volume accumulators of `ngrains`, list counters, status,
+// warnings, IP.Set maxima, statistics; the first overflowing cell at its "none" value) and stores
+// the argument blocks the deferred-cell and grain kernels read. A kernel rather than memset and
+// copy calls, so the sequence records into a HIP graph as kernels only.
+__global__ void scratch_setup_kernel(unsigned char* base, int32_t ngrains, mpf::StepArgs a, mpf::GrainArgs g,
+                                     uint32_t grain_blocks)
+{
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    auto* acc = reinterpret_cast<unsigned long long*>(base + kOffAcc);
+    for (int64_t i = tid; i < static_cast<int64_t>(ngrains) * mpf::kLimbs; i += stride) acc[i] = 0ull;
+    if (tid == 0) {
+        *reinterpret_cast<mpf::StepArgs*>(base + kOffStepArgs) = a;
+        *reinterpret_cast<mpf::GrainArgs*>(base + kOffGrainArgs) = g;
+        auto* first = reinterpret_cast<uint32_t*>(base + kOffFirst);
+        first[0] = 0u;
+        first[1] = grain_blocks;
+        auto* cnt = reinterpret_cast<unsigned*>(base + kOffCount);
+        auto* st = reinterpret_cast<int32_t*>(base + kOffStatus);
+        auto* w = reinterpret_cast<unsigned long long*>(base + kOffWords);
+        for (int i = 0; i < 4; ++i) cnt[i] = 0u;
+        for (int i = 0; i < mpf::kStatusWords; ++i) st[i] = 0;
+        w[0] = ~0ull;                       // first overflowing cell: none
+        for (int i = 1; i < 8; ++i) w[i] = 0ull;  // warnings, IP.Set maxima (4), statistics (2)
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+mpf_status mpf_step_into(mpf_ctx* c, double dt, const void* count_in, const void* index_in, const void* value_in,
+                         void* count_out, void* index_out, void* value_out)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization must precede mpf_step_into");
+    if (!(dt > 0.0)) return c->fail(MPF_ERR_ARG, "dt must be > 0");
+    if (!count_in || !index_in || !value_in || !count_out || !index_out || !value_out)
+        return c->fail(MPF_ERR_ARG, "null state buffer");
+    if (count_in == count_out || index_in == index_out || value_in == value_out)
+        return c->fail(MPF_ERR_ARG, "input and output state buffers must differ");
+    if (c->ncells > kScratchCells)
+        return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: " + std::to_string(c->ncells) + " cells exceed its scratch limit of " +
+                                                std::to_string(kScratchCells) + " cells per call (use mpf_step)");
+    if (c->ngrains > kScratchGrains) return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: too many grains");
+    if (c->dg_avg)
+        return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: driving-force averaging keeps per-pair arrays of the context (use mpf_step)");
+    unsigned char* base = nullptr;
+    if (mpf_status s = scratch_address(c, &base)) return s;
+
+    mpf::StepArgs a = step_args(c, dt, c->cur);
+    a.cnt_in = static_cast<const uint8_t*>(count_in);
+    a.id_in = static_cast<const uint16_t*>(index_in);
+    a.val_in = value_in;
+    a.cnt_out = static_cast<uint8_t*>(count_out);
+    a.id_out = static_cast<uint16_t*>(index_out);
+    a.val_out = value_out;
+    a.vol_acc = reinterpret_cast<unsigned long long*>(base + kOffAcc);
+    a.status = reinterpret_cast<int32_t*>(base + kOffStatus);
+    auto* words = reinterpret_cast<unsigned long long*>(base + kOffWords);
+    a.first_overflow = words;
+    a.warnings = words + 1;
+    a.obs = words + 2;
+    a.work_list = reinterpret_cast<uint32_t*>(base + kOffList);
+    a.work_count = reinterpret_cast<unsigned*>(base + kOffCount);
+    mpf::GrainArgs g = grain_args(c);
+    g.vol_acc = a.vol_acc;
+    g.work_count = a.work_count;
+    g.obs = words + 2;
+    g.stats = words + 6;
+    const unsigned gblocks = grain_blocks(c);
+
+    hipLaunchKernelGGL(scratch_setup_kernel, dim3(64), dim3(256), 0, c->stream, base, c->ngrains, a, g, gblocks);
+    if (mpf_status s = c->hip(hipGetLastError(), "scratch set-up kernel")) return s;
+    const bool full = c->grad || c->dg;
+    mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_step(a, nullptr, nullptr, 1, step_blocks(c), c->fp32, full,
+                                                                     c->uniform, c->stream)), "step kernel");
+    if (!st)
+        st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(reinterpret_cast<const mpf::StepArgs*>(base + kOffStepArgs), 1,
+                                                                 c->deferred_grid, c->fp32, full, c->stream)),
+                    "deferred-cell kernels");
+    if (!st)
+        st = c->hip(static_cast<hipError_t>(mpf::launch_grains(reinterpret_cast<const mpf::GrainArgs*>(base + kOffGrainArgs),
+                                                               reinterpret_cast<const uint32_t*>(base + kOffFirst), 1, gblocks,
+                                                               c->stream)), "grain kernel");
+    if (st) return st;
+    int32_t status[mpf::kStatusWords];
+    st = c->hip(hipMemcpyAsync(status, base + kOffStatus, sizeof(status), hipMemcpyDeviceToHost, c->stream), "status readback");
+    if (!st) st = c->hip(hipStreamSynchronize(c->stream), "stream synchronize");
+    if (st) return st;
+    if (status[mpf::kStatusOverflow]) return c->fail(MPF_SLOT_OVERFLOW, "a cell needed more phase-field slots than the capacity");
+    if (status[mpf::kStatusEmpty]) return c->fail(MPF_EMPTY_NODE, "finalize erased every entry of a cell");
+    if (status[mpf::kStatusVolumeRange]) return c->fail(MPF_ERR_STATE, "phase-field value outside [0, 2) in the volume sum");
+    return MPF_OK;
+}
+
+}  // extern "C"
