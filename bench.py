@@ -2,10 +2,12 @@
 """Benchmark of the MI355X multiphase-field step.
 
 Workload (N=1): 3D normal grain growth on a 256^3 periodic grid, 1000 Voronoi grains
-(uniform-random seeds), isotropic energy 0.24 J/m^2, mobility 4e-9 m^4/(J s), interface width 5
-cells, 16 phase slots per cell, FP64 (the plain drop-in step), dt at the stability limit. A "step"
-is one full MPF time step of the whole grid (IP.Set + increments + normalize + merge + finalize
-+ grain volumes). The same step in FP32 storage/stencil arithmetic is measured beside it.
+(uniform-random seeds, one-cell overlap band as the reference's Voronoi set-up), isotropic energy
+0.24 J/m^2, mobility 4e-9 m^4/(J s), interface width 5 cells, 16 phase slots per cell (a few cells
+hold 9 phase fields between steps 4 and 23, so 8 slots overflow), FP64 (the plain drop-in step),
+dt at the stability limit. A "step" is one full MPF time step of the whole grid (IP.Set +
+increments + normalize + merge + finalize + grain volumes). The same step in FP32
+storage/stencil arithmetic is measured beside it, with its error against the FP64 step.
 
 Prints one JSON line: value = cell-updates/s of the FP64 step (whole job, inputs resident in HBM),
 plus `roofline` for the fused step kernel (HBM bound: algorithmic bytes / average kernel time from
@@ -32,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 NX = NY = NZ = 256
 NGRAINS = 1000
 SEED = 0x0FE2
-# stored-list capacity: 8 overflows in the first steps of this 3D case (9 grains meet in a few
+# stored-list capacity: 8 overflows between steps 4 and 23 of this 3D case (9 grains meet in a few
 # cells); only occupied slots are read or written, so the capacity does not change the traffic
 SLOTS = 16
 SLOTS_FP32 = 15   # the FP32 count byte holds the list length in 4 bits beside the implicit slot
@@ -74,7 +76,7 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
     # second pass with per-kernel HIP events on the library's stream (same number of steps)
     eng.enable_timing(True)
     eng.step(dt, steps)
-    k_step_ms, k_grain_ms = eng.last_timing()
+    k_step_ms, k_rest_ms = eng.last_timing()
     d_mid, d_wide = eng.step_stats()
     total_steps = warmup + 2 * steps
     eng.enable_timing(False)
@@ -88,7 +90,7 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
         "cell_updates_per_s": ncell * steps / wall,
         "ms_per_step": 1e3 * wall / steps,
         "kernel_ms": k_step_ms,
-        "grain_kernel_ms": k_grain_ms,
+        "deferred_and_grain_ms": k_rest_ms,
         "bytes_per_step": b_alg,
         "mean_entries_per_cell": float((c_in.sum() + c_out.sum()) / (2 * ncell)),
         "achieved_gbs": b_alg / (k_step_ms * 1e-3) / 1e9 if k_step_ms > 0 else None,
@@ -97,11 +99,15 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
     }
 
 
-def profiled_traffic(kernel: str = "step_kernel<double, double, false, false, true>"):
+PROFILE = ROOT / "profiles" / "mpf_step_256cube_1000grains.md"
+MAIN_KERNEL = "tile_kernel<double, double, false, false, true>"
+
+
+def profiled_traffic(kernel: str = MAIN_KERNEL):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary (FETCH_SIZE and
     WRITE_SIZE in KiB per dispatch, scaled by the factors the copy kernel gives: 2.00 and 1.00).
     None when the summary does not list the kernel."""
-    path = ROOT / "profiles" / "mpf_step_256cube_1000grains.md"
+    path = PROFILE
     if not path.exists():
         return None
     fetch = write = None
@@ -115,6 +121,51 @@ def profiled_traffic(kernel: str = "step_kernel<double, double, false, false, tr
     if fetch is None or write is None:
         return None
     return fetch * 1024 * 2.0 + write * 1024 * 1.0
+
+
+def fraction_errors(a, b, ngrains: int):
+    """Max and mean |phi_a - phi_b| over every (cell, grain) entry present in either state."""
+    def keys(x):
+        cell = np.repeat(np.arange(x.ncells, dtype=np.int64), x.count)
+        return cell * ngrains + x.index.astype(np.int64)
+    k = np.concatenate([keys(a), keys(b)])
+    v = np.concatenate([a.value, -b.value])
+    _, inv = np.unique(k, return_inverse=True)
+    d = np.abs(np.bincount(inv, weights=v))
+    return float(d.max()), float(d.mean())
+
+
+def run_fp32_accuracy(lists, checks=(100, 500, 1000)):
+    """The FP32 variant against the FP64 step (bit-equal to the CPU oracle) on the benchmark
+    workload: max / mean phase-fraction error over all (cell, grain) entries and the largest grain
+    volume drift (cells, and relative for grains of at least ten nucleus volumes) at each check."""
+    from openphase_amd.engine import Engine, StepConfig
+
+    c64 = StepConfig(NX, NY, NZ, slots=SLOTS)
+    c32 = StepConfig(NX, NY, NZ, precision=1, slots=SLOTS_FP32)
+    dt = c64.max_time_step()
+    e64, e32 = Engine(c64), Engine(c32)
+    for e in (e64, e32):
+        e.set_grains(np.zeros(NGRAINS, np.int32))
+        e.upload(lists)
+        e.finalize_initialization()
+    out, done = {}, 0
+    big_volume = 10.0 * (4.0 / 3.0) * np.pi * 125.0
+    for s in checks:
+        for e in (e64, e32):
+            e.step(dt, s - done)
+        done = s
+        emax, emean = fraction_errors(e64.download(), e32.download(), NGRAINS)
+        v64, _ = e64.grains()
+        v32, _ = e32.grains()
+        drift = np.abs(v32 - v64)
+        big = v64 >= big_volume
+        out[f"step_{s}"] = {"max_phase_fraction_error": emax, "mean_phase_fraction_error": emean,
+                            "max_grain_volume_drift_cells": float(drift.max()),
+                            "max_grain_volume_drift_rel": float((drift[big] / v64[big]).max())}
+    e64.close()
+    e32.close()
+    return out
 
 
 def run_adjoint(steps: int, segment: int):
@@ -269,30 +320,49 @@ def run_dg_average(steps: int):
     return out
 
 
-def cpu_baseline(lists, budget_s: float = 30.0):
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(lists, steps: int = 3):
     """Times the reference OpenPhase build (oracle/_ref/ref_driver, compiled from the reference
-    sources) on this host: same 256^3 state, one step, all available cores (max 16)."""
+    sources) on this host: same 256^3 state, one warm-up step then `steps` timed steps with the
+    reference's four per-call stamps (examples/NormalGG/NormalGG.cpp:76-84), on every core of this
+    process's CPU share (at most 16, the share of one GPU on this machine)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from openphase_amd.engine import StepConfig
 
     if not O.REF_DRIVER.exists():
         return None
-    threads = min(16, len(os.sched_getaffinity(0)))
+    avail = len(os.sched_getaffinity(0))
+    threads = min(16, avail)
     cfg = StepConfig(NX, NY, NZ)
     with tempfile.TemporaryDirectory() as td:
         t0 = time.perf_counter()
-        timing, _ = O.run_reference(td, cfg, lists, np.zeros(NGRAINS, np.int32), None, 1, cfg.max_time_step(), [],
-                                    threads=threads, timeout=max(120, int(budget_s * 10)))
+        timing, _ = O.run_reference(td, cfg, lists, np.zeros(NGRAINS, np.int32), None, steps + 1, cfg.max_time_step(), [],
+                                    threads=threads, timeout=600, untimed=1)
         wall = time.perf_counter() - t0
+    per = {k: timing[k] / steps for k in ("IP.Set", "CalculatePhaseFieldIncrements", "NormalizeIncrements",
+                                          "MergeIncrements")}
     return {
         "value": timing["cell_updates_per_s"],
         "unit": "cell-updates/s",
         "cores": threads,
         "kind": "reference",
-        "sample": f"OpenPhase reference build, same 256^3 1000-grain state, 1 step (IP.Set+Increments+"
-                  f"Normalize+Merge timers), {threads} OpenMP threads; {timing['total_s']:.2f} s step, "
-                  f"{wall:.1f} s incl. file I/O and FinalizeInitialization",
+        "nproc": os.cpu_count(),
+        "cpu_share": avail,
+        "cpu_model": _cpu_model(),
+        "seconds_per_step": {k: round(v, 4) for k, v in per.items()},
+        "sample": f"OpenPhase reference build, same 256^3 1000-grain state, 1 warm-up + {steps} timed steps "
+                  f"(IP.Set / Increments / Normalize / Merge stamps), {threads} OpenMP threads; "
+                  f"{timing['total_s'] / steps:.2f} s per step, {wall:.1f} s incl. file I/O, set-up and warm-up",
     }
 
 
@@ -302,6 +372,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp32-accuracy", action="store_true", help="skip the FP32-vs-FP64 error leg")
+    ap.add_argument("--only-main", action="store_true",
+                    help="only the 256^3 FP64 / FP32 step legs (for profiling runs)")
     ap.add_argument("--adjoint-steps", type=int, default=100, help="rollout length of the adjoint leg")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the ensemble leg")
     ap.add_argument("--dg-steps", type=int, default=200, help="steps of the driving-force averaging leg")
@@ -314,22 +387,28 @@ def main():
     from openphase_amd.microstructure import voronoi_grains
 
     t0 = time.perf_counter()
-    lists = voronoi_grains(NX, NY, NZ, NGRAINS, seed=SEED)
+    lists = voronoi_grains(NX, NY, NZ, NGRAINS, seed=SEED, sides="plus")
     print(f"[bench] microstructure {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     r64 = run_gpu(0, lists, args.steps, args.warmup)
     print(f"[bench] fp64 {r64}", file=sys.stderr, flush=True)
     r32 = run_gpu(1, lists, args.steps, args.warmup)
     print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
-    adj = run_adjoint(args.adjoint_steps, 25)
-    print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
-    ens = run_ensemble(args.ensemble_steps)
-    print(f"[bench] ensemble {ens}", file=sys.stderr, flush=True)
-    dga = run_dg_average(args.dg_steps)
-    print(f"[bench] driving-force averaging {dga}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(lists)
+    acc32 = None if args.no_fp32_accuracy else run_fp32_accuracy(lists)
+    print(f"[bench] fp32 accuracy {acc32}", file=sys.stderr, flush=True)
+    adj = ens = dga = cpu = None
+    if not args.only_main:
+        adj = run_adjoint(args.adjoint_steps, 25)
+        print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
+        ens = run_ensemble(args.ensemble_steps)
+        print(f"[bench] ensemble {ens}", file=sys.stderr, flush=True)
+        dga = run_dg_average(args.dg_steps)
+        print(f"[bench] driving-force averaging {dga}", file=sys.stderr, flush=True)
+    if not (args.no_cpu or args.only_main):
+        cpu = cpu_baseline(lists)
     print(f"[bench] cpu {cpu}", file=sys.stderr, flush=True)
 
     ach = r64["achieved_gbs"]
+    traffic = profiled_traffic()
     out = {
         "metric": "cell-updates/s + HBM GB/s, 256³ MPF step FP32 vs FP64; adjoint steps/s",
         "value": r64["cell_updates_per_s"],
@@ -346,6 +425,7 @@ def main():
         "config": {
             "workload": "3D normal grain growth, 256^3 periodic, 1000 Voronoi grains, isotropic "
                         "sigma=0.24 J/m^2, mu=4e-9, iWidth=5, 16 slots, FP64 plain step",
+            "step_kernel": MAIN_KERNEL,
             "cells": NX * NY * NZ,
             "grains": NGRAINS,
         },
@@ -355,7 +435,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (ach / HBM_PEAK_GBS) if ach else None,
-            "traffic": profiled_traffic(),
+            "traffic": traffic,
+            # HBM bytes the profiled launch actually moved, over this run's kernel time: how far the
+            # kernel is from the bandwidth bound with its real traffic (latency-bound when low)
+            "traffic_gbs": (traffic / (r64["kernel_ms"] * 1e-3) / 1e9) if traffic else None,
+            "traffic_frac": (traffic / (r64["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+            "kernel": MAIN_KERNEL,
+            "profile": str(PROFILE.relative_to(ROOT)),
         },
         "fp32": {
             "cell_updates_per_s": r32["cell_updates_per_s"],
@@ -363,10 +449,11 @@ def main():
             "achieved_gbs": r32["achieved_gbs"],
             "frac": (r32["achieved_gbs"] / HBM_PEAK_GBS) if r32["achieved_gbs"] else None,
             "speedup_vs_fp64": r32["cell_updates_per_s"] / r64["cell_updates_per_s"],
+            "error_vs_fp64": acc32,
         },
         "kernel": {
             "fp64_step_kernel_ms": r64["kernel_ms"],
-            "fp64_grain_kernel_ms": r64["grain_kernel_ms"],
+            "fp64_deferred_and_grain_ms": r64["deferred_and_grain_ms"],
             "fp64_bytes_per_step": r64["bytes_per_step"],
             "fp64_mean_entries_per_cell": r64["mean_entries_per_cell"],
             "fp32_step_kernel_ms": r32["kernel_ms"],

@@ -218,6 +218,15 @@ mpf_status mpf_adjoint_step(mpf_ctx* ctx, double dt, const double* lam_next, dou
 mpf_status mpf_autotune(mpf_ctx* ctx, double dt, int32_t steps, int32_t* xcd_order, int32_t* deferred_grid);
 mpf_status mpf_set_launch_shape(mpf_ctx* ctx, int32_t xcd_order, int32_t deferred_grid);
 
+/* Shape of the LDS-tiled step kernel (no reference counterpart): a block holds a tile_y x tile_z
+ * tile of the (y, z) plane (at most 256 cells, its halo plane at most 396 cells; 1 on reduced
+ * axes; 0 = the default for that axis) and marches through tile_x planes along x (0 = about 8192
+ * blocks). tile_x < 0 selects the per-cell step kernel. Contexts with two or three active axes
+ * start with the default tile; 1D grids always use the per-cell kernel. Results do not depend on
+ * the shape. */
+mpf_status mpf_set_tile_shape(mpf_ctx* ctx, int32_t tile_x, int32_t tile_y, int32_t tile_z);
+mpf_status mpf_get_tile_shape(mpf_ctx* ctx, int32_t* tile_x, int32_t* tile_y, int32_t* tile_z);
+
 /* Ensembles of independent domains (no reference counterpart; calibration runs over many uneven
  * microstructures): the members (initialized contexts, each with its own grid, tables and dt)
  * advance together. Two steps of every member are captured into one HIP graph (one branch per

@@ -32,7 +32,7 @@ EXPORTED = [
     "mpf_max_time_step", "mpf_limiter_warnings", "mpf_overflow_cell", "mpf_device_state",
     "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
     "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_autotune",
-    "mpf_set_launch_shape", "mpf_ensemble_create", "mpf_ensemble_step", "mpf_ensemble_destroy",
+    "mpf_set_launch_shape", "mpf_set_tile_shape", "mpf_get_tile_shape", "mpf_ensemble_create", "mpf_ensemble_step", "mpf_ensemble_destroy",
     "mpf_ensemble_error_message", "mpf_step_into", "mpf_error_message", "mpf_destroy",
 ]
 
@@ -108,6 +108,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_adjoint_step": ([vp, C.c_double, vp, vp, vp], C.c_int),
         "mpf_autotune": ([vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "mpf_set_launch_shape": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "mpf_set_tile_shape": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "mpf_get_tile_shape": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "mpf_ensemble_create": ([C.POINTER(vp), C.c_int32, C.POINTER(vp)], C.c_int),
         "mpf_ensemble_step": ([vp, f64p, C.c_int32], C.c_int),
         "mpf_ensemble_destroy": ([vp], None),

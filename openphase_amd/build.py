@@ -42,14 +42,21 @@ def build_hip() -> Path:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
-    srcs = [CSRC / "mpf_kernels.hip", CSRC / "mpf_adjoint.hip", CSRC / "mpf_dgavg.hip", CSRC / "mpf_capi.cpp"]
+    # (source, object name, extra defines): the per-cell step kernels build as four variants
+    units = [(CSRC / "mpf_kernels.hip", "mpf_kernels", []), (CSRC / "mpf_tile.hip", "mpf_tile", []),
+             (CSRC / "mpf_adjoint.hip", "mpf_adjoint", []), (CSRC / "mpf_dgavg.hip", "mpf_dgavg", []),
+             (CSRC / "mpf_capi.cpp", "mpf_capi", [])]
+    for fp32 in (0, 1):
+        for full in (0, 1):
+            units.append((CSRC / "mpf_kernels_variant.hip", f"mpf_kernels_v{32 if fp32 else 64}_{'full' if full else 'plain'}",
+                          [f"-DMPF_VARIANT_FP32={fp32}", f"-DMPF_VARIANT_FULL={full}"]))
     compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
     procs = []
     objs = []
-    for src in srcs:
-        obj = objdir / (src.stem + ".o")
+    for src, name, defs in units:
+        obj = objdir / (name + ".o")
         objs.append(str(obj))
-        cmd = [hipcc, *compile_flags, "-x", "hip", "-c", str(src), "-o", str(obj)]
+        cmd = [hipcc, *compile_flags, *defs, "-x", "hip", "-c", str(src), "-o", str(obj)]
         print("+", " ".join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
     if any(p.wait() != 0 for p in procs):

@@ -70,6 +70,9 @@ struct StepArgs {
     // the number of blocks of the grid-stride deferred-cell kernels
     int32_t xcd_order;
     int32_t deferred_grid;
+    // LDS-tiled step kernel (mpf_tile.hip): tile of tile_y x tile_z cells of the (y, z) plane
+    // marching through tile_x planes along x; tile_x = 0 selects the per-cell step kernel
+    int32_t tile_x, tile_y, tile_z;
 };
 
 struct GrainArgs {
@@ -107,6 +110,10 @@ constexpr int kAdjSlots = 8;     // merged entries per cell the adjoint handles
 int launch_adjoint(const AdjointArgs& a, void* stream);
 
 constexpr int kGrainBlock = 256;                     // threads per block of the grain kernel
+constexpr int kTileHalo = 396;                       // cells of a tile plane with its halo, at most (6 x 66)
+
+// LDS-tiled step kernel over one domain (mpf_tile.hip); `blocks` = tiles x x-chunks.
+int launch_tile(const StepArgs& a, unsigned blocks, bool fp32, bool full, bool uni, void* stream);
 
 // Driving-force averaging (mpf_dgavg.hip): CollectAverage into `tmp`, DistributeAverage into
 // `avg` (= s.dg_avg), both [pairs][ncells] with pair a < b of the own entries at b(b-1)/2 + a.
@@ -136,6 +143,18 @@ int launch_dg_average(const AvgArgs& a, bool fp32, void* stream);
 int launch_step(const StepArgs& one, const StepArgs* args, const uint32_t* first, int ndom, unsigned blocks, bool fp32,
                 bool full, bool uni, void* stream);
 int launch_deferred(const StepArgs* args, int ndom, int per_domain, bool fp32, bool full, void* stream);
+// the variants behind launch_step / launch_deferred, one translation unit each
+#define MPF_VARIANT_LAUNCHERS(NS)                                                                            \
+    namespace NS {                                                                                           \
+    int launch_step(const StepArgs& one, const StepArgs* args, const uint32_t* first, int ndom, unsigned blocks, \
+                    bool uni, void* stream);                                                                 \
+    int launch_deferred(const StepArgs* args, int ndom, int per_domain, void* stream);                       \
+    }
+MPF_VARIANT_LAUNCHERS(v64_plain)
+MPF_VARIANT_LAUNCHERS(v64_full)
+MPF_VARIANT_LAUNCHERS(v32_plain)
+MPF_VARIANT_LAUNCHERS(v32_full)
+#undef MPF_VARIANT_LAUNCHERS
 int launch_finalize_init(const StepArgs& a, bool fp32, void* stream);
 int launch_grains(const GrainArgs* args, const uint32_t* first, int ndom, unsigned blocks, void* stream);
 int launch_copy(const void* src, void* dst, int64_t bytes, void* stream);

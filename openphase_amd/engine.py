@@ -291,6 +291,15 @@ class Engine:
         self._check(self.lib.mpf_autotune(self.h, float(dt), int(steps), C.byref(x), C.byref(g)), "mpf_autotune")
         return x.value, g.value
 
+    def set_tile_shape(self, tile_x: int, tile_y: int = 0, tile_z: int = 0) -> None:
+        """LDS-tiled step kernel shape (mpf_set_tile_shape); tile_x < 0 selects the per-cell kernel."""
+        self._check(self.lib.mpf_set_tile_shape(self.h, int(tile_x), int(tile_y), int(tile_z)), "mpf_set_tile_shape")
+
+    def tile_shape(self):
+        x, y, z = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.mpf_get_tile_shape(self.h, C.byref(x), C.byref(y), C.byref(z)), "mpf_get_tile_shape")
+        return x.value, y.value, z.value
+
     def set_launch_shape(self, xcd_order: int, deferred_grid: int) -> None:
         self._check(self.lib.mpf_set_launch_shape(self.h, int(xcd_order), int(deferred_grid)), "mpf_set_launch_shape")
 

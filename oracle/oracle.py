@@ -277,7 +277,7 @@ def write_opi(path, cfg, threads: int = 1, dt: float | None = None, nsteps: int 
 
 
 def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dumps, threads=1, timeout=3600,
-                  write_files=False):
+                  write_files=False, untimed=0):
     """Runs the real reference (oracle/_ref/ref_driver) on `lists`; returns the parsed timing JSON
     and {step: (Lists, volume, volume_ratio)} for the dump steps. With write_files, each dump
     step's entry also holds the bytes of the reference's own output files: {"dat": PhaseField::Write,
@@ -296,6 +296,8 @@ def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dum
            repr(float(dt)), str(workdir / "out"), ",".join(str(s) for s in dumps), "--threads", str(threads)]
     if cfg.dg_mode is not None:
         cmd.append("--driving-force")
+    if untimed:
+        cmd += ["--untimed", str(int(untimed))]
     if write_files:
         cmd.append("--write-files")
         (workdir / "VTK").mkdir(exist_ok=True)

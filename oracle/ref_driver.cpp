@@ -11,6 +11,8 @@
 //
 // usage: ref_driver <input.opi> <state.dat> <grains.bin> <nsteps> <dt> <outprefix> <dumpsteps,...>
 //        [--driving-force] [--threads N] [--write-files] [--normalize-only <pairs.bin>]
+//        [--untimed N]
+// --untimed N: the first N steps run but are left out of the per-call timers (warm-up).
 // --normalize-only: after FinalizeInitialization, adds the pair increments of pairs.bin to
 // Phi.FieldsDot with NodeAB::add_asym1 (uint32 ncells; per cell uint32 linear index (x outer,
 // z inner), uint32 npairs, npairs x {uint32 a; uint32 b; double value}), runs only
@@ -96,7 +98,7 @@ int main(int argc, char* argv[])
     }
     bool with_dg = false, write_files = false;
     std::string normalize_only;
-    int threads = 1;
+    int threads = 1, untimed = 0;
     for (int a = 8; a < argc; ++a)
     {
         std::string s = argv[a];
@@ -104,6 +106,7 @@ int main(int argc, char* argv[])
         if (s == "--write-files") write_files = true;
         if (s == "--threads" && a + 1 < argc) threads = std::atoi(argv[++a]);
         if (s == "--normalize-only" && a + 1 < argc) normalize_only = argv[++a];
+        if (s == "--untimed" && a + 1 < argc) untimed = std::atoi(argv[++a]);
     }
     omp_set_num_threads(threads);
 
@@ -205,7 +208,7 @@ int main(int argc, char* argv[])
         double t4 = omp_get_wtime();
         Phi.MergeIncrements(BC, dt);
         double t5 = omp_get_wtime();
-        t_ip += t1 - t0; t_do += t2 - t1; t_dg += t3 - t2; t_norm += t4 - t3; t_merge += t5 - t4;
+        if (step > untimed) { t_ip += t1 - t0; t_do += t2 - t1; t_dg += t3 - t2; t_norm += t4 - t3; t_merge += t5 - t4; }
         max_dt.push_back(IP.ReportMaximumTimeStep());
         if (dumps.count(step)) dump_state(Phi, prefix, step);
         if (dumps.count(step) && write_files)
@@ -220,8 +223,8 @@ int main(int argc, char* argv[])
     std::printf("{\"cells\": %ld, \"steps\": %d, \"threads\": %d, \"IP.Set\": %.6f, \"CalculatePhaseFieldIncrements\": %.6f, "
                 "\"DrivingForce\": %.6f, \"NormalizeIncrements\": %.6f, \"MergeIncrements\": %.6f, \"total_s\": %.6f, "
                 "\"cell_updates_per_s\": %.6e, \"max_dt_init\": %.17g, \"max_dt\": [",
-                cells, nsteps, threads, t_ip, t_do, t_dg, t_norm, t_merge, total,
-                total > 0 ? double(cells) * nsteps / total : 0.0, max_dt_init);
+                cells, nsteps - untimed, threads, t_ip, t_do, t_dg, t_norm, t_merge, total,
+                total > 0 ? double(cells) * (nsteps - untimed) / total : 0.0, max_dt_init);
     for (size_t i = 0; i < max_dt.size(); ++i) std::printf("%s%.17g", i ? ", " : "", max_dt[i]);
     std::printf("]}\n");
     delete dG;

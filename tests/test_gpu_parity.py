@@ -26,11 +26,27 @@ def _all_grains_full_size(vrs):
     return all(np.all(v == 1.0) for v in vrs)
 
 
+# step kernels: the LDS-tiled one (default shape, and a shape with partial tiles and uneven x
+# chunks) and the per-cell one
+KERNELS = ["tile", "tile-odd", "cell"]
+
+
+def _kernel(eng, kernel):
+    if kernel == "cell":
+        eng.set_tile_shape(-1)
+    elif kernel == "tile-odd":
+        nx, ny, nz = eng.cfg.nx, eng.cfg.ny, eng.cfg.nz
+        eng.set_tile_shape(3, min(5, ny) if ny else 1, min(6, nz) if nz else 1)
+    return eng
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name", CASES)
-def test_fp64_matches_golden(name):
+def test_fp64_matches_golden(name, kernel):
     case = Case(name)
-    print(f"case {name} cfg {case.cfg.shape}", flush=True)
-    eng = case.setup(Engine(case.cfg))
+    print(f"case {name} cfg {case.cfg.shape} kernel {kernel}", flush=True)
+    eng = _kernel(Engine(case.cfg), kernel)
+    case.setup(eng)
     iso = case.cfg.energy_model is None
     exact = iso and _all_grains_full_size([case.expect[s][2] for s in case.dumps])
     # ReportMaximumTimeStep: DBL_MAX before the first IP.Set, as the reference
@@ -55,15 +71,16 @@ def test_fp64_matches_golden(name):
     eng.close()
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("shape,ngrains,steps", [((64, 64, 0), 24, 150), ((48, 48, 48), 20, 25), ((40, 0, 36), 12, 60)])
-def test_fp64_matches_oracle(shape, ngrains, steps):
+def test_fp64_matches_oracle(shape, ngrains, steps, kernel):
     nx, ny, nz = shape
     cfg = StepConfig(nx, ny, nz, slots=16)
     lists = voronoi_grains(nx, ny, nz, ngrains, seed=100 + ngrains)
     phase = np.zeros(ngrains, np.int32)
     dt = cfg.max_time_step()
-    print(f"oracle parity {shape} grains {ngrains} steps {steps}", flush=True)
-    eng = Engine(cfg)
+    print(f"oracle parity {shape} grains {ngrains} steps {steps} kernel {kernel}", flush=True)
+    eng = _kernel(Engine(cfg), kernel)
     orc = O.Oracle(cfg)
     for e in (eng, orc):
         e.set_grains(phase)
