@@ -136,7 +136,7 @@ __device__ __forceinline__ void tile_derivatives(const StepArgs& A, const uint32
 }
 
 template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
-__global__ __launch_bounds__(kBlock) void tile_kernel(const StepArgs A)
+__global__ __launch_bounds__(kBlock, sizeof(TV) == 4 && !GRAD ? 4 : 1) void tile_kernel(const StepArgs A)
 {
     __shared__ uint32_t s_info[3][kTileHalo];
     __shared__ uint32_t s_mask[3][kTileHalo];
