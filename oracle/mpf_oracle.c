@@ -95,6 +95,11 @@ struct orc_ctx {
     orc_force* force;
     long* force_off;
     long force_cap;
+    /* per-step switches of the driving force (orc_set_step_terms) */
+    int dg_step, avg_step;
+    /* lists the last step started from (orc_keep_step_start) */
+    int keep_ip, ip_valid;
+    orc_node* ipnodes;
 };
 
 static long cell_index(const orc_ctx* c, long i, long j, long k)
@@ -219,6 +224,8 @@ int orc_create(const orc_params* p, int32_t ngrains, const int32_t* grain_phase,
         }
         c->force_off = (long*)calloc((size_t)c->ncells + 1, sizeof(long));
     }
+    c->dg_step = 1;
+    c->avg_step = 1;
     c->emodel = (int32_t*)calloc(np2, sizeof(int32_t));
     c->need_grad = 0;
     for (size_t i = 0; i < np2; ++i) {
@@ -268,7 +275,7 @@ void orc_destroy(orc_ctx* c)
     free(c->volume); free(c->maxvolume); free(c->refvolume); free(c->vr);
     free(c->exist); free(c->stage);
     free(c->nodes); free(c->tmp);
-    free(c->force); free(c->force_off);
+    free(c->force); free(c->force_off); free(c->ipnodes);
     free(c);
 }
 
@@ -284,6 +291,7 @@ void orc_set_threads(int32_t n)
 int orc_load(orc_ctx* c, const int32_t* count, const int32_t* index, const double* value)
 {
     long off = 0;
+    c->ip_valid = 0;
     for (long i = 0; i < c->ncells; ++i) {
         orc_node* nd = &c->nodes[i];
         if (count[i] < 0 || count[i] > ORC_CAP) return ORC_ERR_CAPACITY;
@@ -395,6 +403,41 @@ static double max_energy_of(const orc_ctx* c, int pa, int pb)
     int k = pa * c->nphases + pb;
     if (c->emodel[k] == ORC_EN_CUBIC) return c->sigma[k] * (1.0 + c->eps[k] * 1.41);
     return c->sigma[k];
+}
+
+/* Interface energy InterfaceProperties::SetSR gives the pair (alpha = ea, beta = eb) of a
+ * wide-interface node (src/InterfaceProperties.cpp:620-661): the phase-pair energy (ISO), or the
+ * cubic energy of each solid grain's interface normal weighted by phi (1 - phi), divided by the
+ * sum of the weights, MaxEnergy when neither grain weighs (CUBIC). */
+static double ip_pair_energy(const orc_ctx* c, const orc_entry* ea, const orc_entry* eb)
+{
+    int pa = c->gphase[ea->idx], pb = c->gphase[eb->idx];
+    int k = pa * c->nphases + pb;
+    if (c->emodel[k] == ORC_EN_ISO) return c->sigma[k];
+    double en = 0.0, wa = 0.0, wb = 0.0;
+    if (c->gsolid[ea->idx]) {
+        wa = fabs(ea->val * (1.0 - ea->val));
+        if (wa > DBL_EPSILON) {
+            double nrm[3];
+            interface_orientation(c, ea, nrm);
+            en += cubic_energy(c->sigma[k], c->eps[k], nrm) * wa;
+        } else {
+            wa = 0.0;
+        }
+    }
+    if (c->gsolid[eb->idx]) {
+        wb = fabs(eb->val * (1.0 - eb->val));
+        if (wb > DBL_EPSILON) {
+            double nrm[3];
+            interface_orientation(c, eb, nrm);
+            en += cubic_energy(c->sigma[k], c->eps[k], nrm) * wb;
+        } else {
+            wb = 0.0;
+        }
+    }
+    if (wa + wb > DBL_EPSILON) en /= wa + wb;
+    else en = max_energy_of(c, pa, pb);
+    return en;
 }
 
 /* PhaseField::NormalizeIncrementsSR (src/PhaseField.cpp:1351-1567) on one node: the single-pair
@@ -544,34 +587,7 @@ static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
         const orc_entry* eb = &nd->e[b];
         int pa = c->gphase[ea->idx], pb = c->gphase[eb->idx];
         int k = pa * c->nphases + pb;
-        double en = 0.0;
-        if (c->emodel[k] == ORC_EN_ISO) {
-            en = c->sigma[k];
-        } else {
-            double wa = 0.0, wb = 0.0;
-            if (c->gsolid[ea->idx]) {
-                wa = fabs(ea->val * (1.0 - ea->val));
-                if (wa > DBL_EPSILON) {
-                    double nrm[3];
-                    interface_orientation(c, ea, nrm);
-                    en += cubic_energy(c->sigma[k], c->eps[k], nrm) * wa;
-                } else {
-                    wa = 0.0;
-                }
-            }
-            if (c->gsolid[eb->idx]) {
-                wb = fabs(eb->val * (1.0 - eb->val));
-                if (wb > DBL_EPSILON) {
-                    double nrm[3];
-                    interface_orientation(c, eb, nrm);
-                    en += cubic_energy(c->sigma[k], c->eps[k], nrm) * wb;
-                } else {
-                    wb = 0.0;
-                }
-            }
-            if (wa + wb > DBL_EPSILON) en /= wa + wb;
-            else en = max_energy_of(c, pa, pb);
-        }
+        double en = ip_pair_energy(c, ea, eb);
         double mo = c->mu[k];
         if (en != 0.0 || mo != 0.0) {
             sig[a][b] = sig[b][a] = en;
@@ -626,9 +642,9 @@ static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
      * node's Force entry (dg_average_pass), and DrivingForce::MergePhaseFieldIncrementsSR
      * (src/DrivingForce.cpp:497-602), Limiting on. No grain is a Seed (grains enter Stable and
      * only nucleation makes seeds), so the 1e-6 seed term of :528-536 never applies. */
-    if (c->dgval && nd->flag == 2) {
+    if (c->dgval && c->dg_step && nd->flag == 2) {
         const double pref = ORC_PI / eta;
-        const orc_force* fe = c->avg_on ? &c->force[c->force_off[nd - c->nodes]] : NULL;
+        const orc_force* fe = c->avg_on && c->avg_step ? &c->force[c->force_off[nd - c->nodes]] : NULL;
         for (int a = 0; a < n - 1; ++a)
         for (int b = a + 1; b < n; ++b) {
             int pa = c->gphase[nd->e[a].idx], pb = c->gphase[nd->e[b].idx];
@@ -827,6 +843,7 @@ static int finalize_sr(orc_ctx* c)
 
 int orc_finalize_initialization(orc_ctx* c)
 {
+    c->ip_valid = 0;
     return finalize_sr(c);
 }
 
@@ -946,7 +963,12 @@ static void dg_average_pass(orc_ctx* c, int second)
 int orc_step(orc_ctx* c, double dt, int32_t nsteps)
 {
     for (int s = 0; s < nsteps; ++s) {
-        if (c->avg_on) {
+        if (c->keep_ip) {
+            /* the lists this step's IP.Set sees (neighbour-merged, with their derivatives) */
+            memcpy(c->ipnodes, c->nodes, (size_t)c->ncells * sizeof(orc_node));
+            c->ip_valid = 1;
+        }
+        if (c->avg_on && c->avg_step && c->dg_step) {
             /* DrivingForce::Average (src/DrivingForce.cpp:242-255) on the step-start lists */
             int st = dg_set_forces(c);
             if (st) return st;
@@ -1073,5 +1095,100 @@ int orc_normalize_pairs(int32_t n, const int32_t* idx, const double* val, int32_
     }
     *npairs_io = nd_dot;
     if (warnings) *warnings = w;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Driving-force switches and the interface energy                                            */
+
+int orc_set_step_terms(orc_ctx* c, int32_t driving_force, int32_t average)
+{
+    if (!c) return ORC_ERR_ARG;
+    c->dg_step = driving_force != 0;
+    c->avg_step = average != 0;
+    return ORC_OK;
+}
+
+int orc_keep_step_start(orc_ctx* c, int32_t on)
+{
+    if (!c) return ORC_ERR_ARG;
+    if (on && !c->ipnodes) {
+        c->ipnodes = (orc_node*)calloc((size_t)c->ncells, sizeof(orc_node));
+        if (!c->ipnodes) return ORC_ERR_NOMEM;
+    }
+    c->keep_ip = on != 0;
+    c->ip_valid = 0;
+    return ORC_OK;
+}
+
+/* DoubleObstacle::PointEnergy (src/DoubleObstacle.cpp:903-923) summed in storage order, as
+ * DoubleObstacle::Energy / AverageEnergyDensity (:953-967, :989-1005) sum it on one thread: over
+ * every pair alpha < beta of a wide-interface node's list (zero-valued appended entries included)
+ * 4 E (phi_a phi_b - Eta^2/Pi^2 grad_a . grad_b) / Eta, with the gradients of the current lists
+ * (PhaseField::CalculateDerivativesSR, src/PhaseField.cpp:588-603) and E the energy
+ * InterfaceProperties::Properties holds for the pair: what the last IP.Set gave it on the lists
+ * of `ipn` (NULL: no IP.Set yet, every energy 0; a pair absent there, or a node that was not a
+ * wide interface, reads 0, NodeIP::get_energy include/Containers/NodeIP.h:379-388). */
+static double energy_sum(const orc_ctx* c, const orc_node* ipn)
+{
+    const double eta = c->p.iwidth * c->p.dx;
+    const double pref = eta * eta / (ORC_PI * ORC_PI);
+    double sum = 0.0;
+    for (long ci = 0; ci < c->ncells; ++ci) {
+        const orc_node* nd = &c->nodes[ci];
+        double e = 0.0;
+        if (nd->flag != 0 && ipn) {
+            double g[ORC_CAP][3];
+            memset(g, 0, sizeof(g));
+            long i = ci / ((long)c->ny * c->nz), j = (ci / c->nz) % c->ny, k = ci % c->nz;
+            for (int s = 0; s < c->ngs; ++s) {
+                const orc_stencil* st = &c->gs[s];
+                long nb = cell_index(c, wrap(i + st->di, c->nx), wrap(j + st->dj, c->ny), wrap(k + st->dk, c->nz));
+                const orc_node* nn = &c->nodes[nb];
+                for (int q = 0; q < nn->n; ++q) {
+                    if (nn->e[q].val == 0.0) continue;
+                    for (int t = 0; t < nd->n; ++t)
+                        if (nd->e[t].idx == nn->e[q].idx) {
+                            g[t][0] = g[t][0] + st->gx * nn->e[q].val;
+                            g[t][1] = g[t][1] + st->gy * nn->e[q].val;
+                            g[t][2] = g[t][2] + st->gz * nn->e[q].val;
+                            break;
+                        }
+                }
+            }
+            const orc_node* ip = &ipn[ci];
+            for (int a = 0; a < nd->n; ++a)
+            for (int b = a + 1; b < nd->n; ++b) {
+                double en = 0.0;
+                if (ip->flag != 0) {
+                    int pa = -1, pb = -1;
+                    for (int t = 0; t < ip->n; ++t) {
+                        if (ip->e[t].idx == nd->e[a].idx) pa = t;
+                        if (ip->e[t].idx == nd->e[b].idx) pb = t;
+                    }
+                    if (pa >= 0 && pb >= 0)
+                        en = pa < pb ? ip_pair_energy(c, &ip->e[pa], &ip->e[pb]) : ip_pair_energy(c, &ip->e[pb], &ip->e[pa]);
+                }
+                const double dot = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+                e += 4.0 * en * (nd->e[a].val * nd->e[b].val - pref * dot) / eta;
+            }
+        }
+        sum += e;
+    }
+    return sum;
+}
+
+int orc_interface_energy(const orc_ctx* c, int32_t ip_state, double* sum)
+{
+    if (!c || !sum) return ORC_ERR_ARG;
+    const orc_node* ipn = NULL;
+    if (ip_state == ORC_IP_CURRENT) ipn = c->nodes;
+    else if (ip_state == ORC_IP_PREVIOUS) {
+        if (!c->keep_ip || !c->ip_valid) return ORC_ERR_ARG;
+        ipn = c->ipnodes;
+    } else if (ip_state != ORC_IP_NONE) {
+        return ORC_ERR_ARG;
+    }
+    *sum = energy_sum(c, ipn);
     return ORC_OK;
 }

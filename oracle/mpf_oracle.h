@@ -94,6 +94,22 @@ int orc_normalize_pairs(int32_t n, const int32_t* idx, const double* val, int32_
 /* InterfaceProperties::ReportMaximumTimeStep for the last IP.Set (src/InterfaceProperties.cpp:523-556). */
 double orc_max_time_step(const orc_ctx* c);
 void orc_set_threads(int32_t n);
+
+/* Per-step switches of the driving force. The reference applies it in the steps whose driver
+ * calls UserDrivingForce::SetDrivingForce, DrivingForce::Average and
+ * DrivingForce::MergePhaseFieldIncrements: driving_force 0 leaves it out of the following steps;
+ * average 0 takes the raw value (DrivingForce::SkipAverage) in a context created with averaging. */
+int orc_set_step_terms(orc_ctx* c, int32_t driving_force, int32_t average);
+/* Keep a copy of the lists every following step starts from (what that step's IP.Set sees), for
+ * orc_interface_energy(c, ORC_IP_PREVIOUS, ...). */
+int orc_keep_step_start(orc_ctx* c, int32_t on);
+/* State the interface energies of InterfaceProperties come from: no IP.Set yet, the current
+ * lists, or the lists the last step started from (IP.Set -> step -> energy). */
+enum orc_ip_state { ORC_IP_NONE = 0, ORC_IP_CURRENT = 1, ORC_IP_PREVIOUS = 2 };
+/* Sum over all cells of DoubleObstacle::PointEnergy (src/DoubleObstacle.cpp:903-923), in storage
+ * order: DoubleObstacle::AverageEnergyDensity = sum / cells (:989-1005), Energy = sum * cell
+ * volume (:953-967). */
+int orc_interface_energy(const orc_ctx* c, int32_t ip_state, double* sum);
 /* Longest list seen after the neighbour merge (incl. zero-valued entries) and after finalize. */
 void orc_list_stats(const orc_ctx* c, int32_t* max_merged, int32_t* max_finalized);
 

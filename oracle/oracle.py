@@ -61,6 +61,9 @@ def load():
         lib.orc_set_threads.argtypes = [C.c_int32]
         lib.orc_normalize_pairs.argtypes = [C.c_int32, i32p, f64p, i32p, i32p, i32p, f64p, C.c_double,
                                             C.POINTER(C.c_int64)]
+        lib.orc_set_step_terms.argtypes = [vp, C.c_int32, C.c_int32]
+        lib.orc_keep_step_start.argtypes = [vp, C.c_int32]
+        lib.orc_interface_energy.argtypes = [vp, C.c_int32, f64p]
         _lib = lib
     return _lib
 
@@ -145,6 +148,11 @@ class Oracle:
         if st:
             raise RuntimeError(f"orc_step failed: {st}")
 
+    @staticmethod
+    def _check(st, what):
+        if st:
+            raise RuntimeError(f"orc_{what} failed: {st}")
+
     def download(self, with_flags=False):
         from openphase_amd.microstructure import Lists
 
@@ -177,6 +185,26 @@ class Oracle:
     def max_time_step(self):
         """InterfaceProperties::ReportMaximumTimeStep from the maxima the last step's IP.Set saw."""
         return float(self.lib.orc_max_time_step(self.h))
+
+    def set_step_terms(self, driving_force: bool = True, average: bool = True):
+        """Driving force in the following steps or not (the reference's driver calls
+        UserDrivingForce::SetDrivingForce / DrivingForce::Average / MergePhaseFieldIncrements),
+        averaged or not (DrivingForce::SkipAverage) when the configuration averages."""
+        self._check(self.lib.orc_set_step_terms(self.h, int(bool(driving_force)), int(bool(average))), "set_step_terms")
+
+    def keep_step_start(self, on: bool = True):
+        """Keep the lists each following step starts from, for interface_energy("previous")."""
+        self._check(self.lib.orc_keep_step_start(self.h, int(bool(on))), "keep_step_start")
+
+    IP_STATES = {"none": 0, "current": 1, "previous": 2}
+
+    def interface_energy(self, ip: str = "previous") -> float:
+        """Sum over cells of DoubleObstacle::PointEnergy with the interface energies of the last
+        IP.Set: on no state ("none"), the current lists ("current") or the lists the last step
+        started from ("previous", needs keep_step_start before that step)."""
+        out = C.c_double(0.0)
+        self._check(self.lib.orc_interface_energy(self.h, self.IP_STATES[ip], C.byref(out)), "interface_energy")
+        return out.value
 
     def close(self):
         if self.h:
@@ -277,11 +305,14 @@ def write_opi(path, cfg, threads: int = 1, dt: float | None = None, nsteps: int 
 
 
 def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dumps, threads=1, timeout=3600,
-                  write_files=False, untimed=0):
+                  write_files=False, untimed=0, energy=False, dg_every=1):
     """Runs the real reference (oracle/_ref/ref_driver) on `lists`; returns the parsed timing JSON
     and {step: (Lists, volume, volume_ratio)} for the dump steps. With write_files, each dump
     step's entry also holds the bytes of the reference's own output files: {"dat": PhaseField::Write,
-    "grains": GrainsProperties::Write, "vtk": PhaseField::WriteVTK}."""
+    "grains": GrainsProperties::Write, "vtk": PhaseField::WriteVTK}. With energy, the JSON holds
+    DoubleObstacle::AverageEnergyDensity after every step ("energy") and after one more IP.Set on
+    the final state ("energy_ip_current"); dg_every N applies UserDrivingForce::SetDrivingForce
+    only in steps divisible by N."""
     import json
 
     from openphase_amd import rawio
@@ -298,6 +329,10 @@ def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dum
         cmd.append("--driving-force")
     if untimed:
         cmd += ["--untimed", str(int(untimed))]
+    if energy:
+        cmd.append("--energy")
+    if dg_every != 1:
+        cmd += ["--dg-every", str(int(dg_every))]
     if write_files:
         cmd.append("--write-files")
         (workdir / "VTK").mkdir(exist_ok=True)

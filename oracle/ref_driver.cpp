@@ -11,6 +11,7 @@
 //
 // usage: ref_driver <input.opi> <state.dat> <grains.bin> <nsteps> <dt> <outprefix> <dumpsteps,...>
 //        [--driving-force] [--threads N] [--write-files] [--normalize-only <pairs.bin>]
+//        [--energy] [--dg-every N]
 //        [--untimed N]
 // --untimed N: the first N steps run but are left out of the per-call timers (warm-up).
 // --normalize-only: after FinalizeInitialization, adds the pair increments of pairs.bin to
@@ -26,6 +27,10 @@
 // and prints one JSON line with per-call wall times, InterfaceProperties::ReportMaximumTimeStep
 // before the first IP.Set ("max_dt_init") and after the IP.Set of every step ("max_dt", one per
 // step). The limiter's plausibility warnings go to stderr as the reference writes them.
+// --energy: also DoubleObstacle::AverageEnergyDensity after every step ("energy") and after one
+// more IP.Set on the final state ("energy_ip_current").
+// --dg-every N: with --driving-force, UserDrivingForce::SetDrivingForce only in steps divisible
+// by N (DrivingForce::Clear, Average and MergePhaseFieldIncrements in every step).
 
 #include "Settings.h"
 #include "RunTimeControl.h"
@@ -96,9 +101,9 @@ int main(int argc, char* argv[])
         std::string tok;
         while (std::getline(ss, tok, ',')) if (!tok.empty()) dumps.insert(std::atoi(tok.c_str()));
     }
-    bool with_dg = false, write_files = false;
+    bool with_dg = false, write_files = false, energy = false;
     std::string normalize_only;
-    int threads = 1, untimed = 0;
+    int threads = 1, untimed = 0, dg_every = 1;
     for (int a = 8; a < argc; ++a)
     {
         std::string s = argv[a];
@@ -107,6 +112,8 @@ int main(int argc, char* argv[])
         if (s == "--threads" && a + 1 < argc) threads = std::atoi(argv[++a]);
         if (s == "--normalize-only" && a + 1 < argc) normalize_only = argv[++a];
         if (s == "--untimed" && a + 1 < argc) untimed = std::atoi(argv[++a]);
+        if (s == "--energy") energy = true;
+        if (s == "--dg-every" && a + 1 < argc) dg_every = std::atoi(argv[++a]);
     }
     omp_set_num_threads(threads);
 
@@ -186,7 +193,7 @@ int main(int argc, char* argv[])
     }
 
     const double max_dt_init = IP.ReportMaximumTimeStep();
-    std::vector<double> max_dt;
+    std::vector<double> max_dt, energies;
     double t_ip = 0, t_do = 0, t_norm = 0, t_merge = 0, t_dg = 0;
     for (int step = 1; step <= nsteps; ++step)
     {
@@ -199,7 +206,7 @@ int main(int argc, char* argv[])
         double t2 = omp_get_wtime();
         if (with_dg)
         {
-            UDF->SetDrivingForce(Phi, *dG);
+            if (step % dg_every == 0) UDF->SetDrivingForce(Phi, *dG);
             dG->Average(Phi, BC);
             dG->MergePhaseFieldIncrements(Phi, IP);
         }
@@ -210,6 +217,7 @@ int main(int argc, char* argv[])
         double t5 = omp_get_wtime();
         if (step > untimed) { t_ip += t1 - t0; t_do += t2 - t1; t_dg += t3 - t2; t_norm += t4 - t3; t_merge += t5 - t4; }
         max_dt.push_back(IP.ReportMaximumTimeStep());
+        if (energy) energies.push_back(DO.AverageEnergyDensity(Phi, IP));
         if (dumps.count(step)) dump_state(Phi, prefix, step);
         if (dumps.count(step) && write_files)
         {
@@ -226,7 +234,17 @@ int main(int argc, char* argv[])
                 cells, nsteps - untimed, threads, t_ip, t_do, t_dg, t_norm, t_merge, total,
                 total > 0 ? double(cells) * (nsteps - untimed) / total : 0.0, max_dt_init);
     for (size_t i = 0; i < max_dt.size(); ++i) std::printf("%s%.17g", i ? ", " : "", max_dt[i]);
-    std::printf("]}\n");
+    std::printf("]");
+    if (energy)
+    {
+        // DoubleObstacle::AverageEnergyDensity after every step (IP from the step's IP.Set), then
+        // once more after a fresh IP.Set on the final state
+        std::printf(", \"energy\": [");
+        for (size_t i = 0; i < energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", energies[i]);
+        IP.Set(Phi, BC);
+        std::printf("], \"energy_ip_current\": %.17g", DO.AverageEnergyDensity(Phi, IP));
+    }
+    std::printf("}\n");
     delete dG;
     delete UDF;
     return 0;

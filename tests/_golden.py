@@ -44,6 +44,13 @@ class Case:
         self.max_dt_init = float(z["max_dt_init"])
         self.max_dt = z["max_dt"]
         self.limiter_warnings = int(z["limiter_warnings"])
+        # DoubleObstacle::AverageEnergyDensity after step s (index s - 1), with the interface
+        # energies of that step's IP.Set, and after a fresh IP.Set on the last step's state
+        self.energy = z["energy"]
+        self.energy_ip_current = float(z["energy_ip_current"])
+        self.ncells = int(np.prod(storage_shape(self.cfg.nx, self.cfg.ny, self.cfg.nz)))
+        # the driver sets the driving force only in steps divisible by dg_every
+        self.dg_every = int(z["dg_every"])
         self.expect = {
             s: (Lists(shape, z[f"s{s}_count"], z[f"s{s}_index"], z[f"s{s}_value"]), z[f"s{s}_volume"], z[f"s{s}_vr"])
             for s in self.dumps
@@ -53,13 +60,20 @@ class Case:
         engine.set_grains(self.phase, self.rotation)
         engine.upload(self.init)
         engine.finalize_initialization()
+        if hasattr(engine, "keep_step_start"):
+            engine.keep_step_start(True)
         return engine
 
     def run(self, engine):
         """Yields (step, lists, volume, volume_ratio) at every dump step."""
         done = 0
         for s in self.dumps:
-            if s > done:
+            if s > done and self.dg_every != 1:
+                for t in range(done + 1, s + 1):
+                    engine.set_step_terms(t % self.dg_every == 0)
+                    engine.step(self.dt, 1)
+                done = s
+            elif s > done:
                 engine.step(self.dt, s - done)
                 done = s
             vol, vr = engine.grains()

@@ -28,7 +28,7 @@ HERE = Path(__file__).resolve().parent
 
 
 def cases():
-    """(name, cfg, ngrains, seed, grain_phase, quaternions, nsteps, dumps, dt_factor)."""
+    """(name, cfg, ngrains, seed, grain_phase, quaternions, nsteps, dumps, dt_factor[, dg_every])."""
     out = []
     # cfg1-like: 2D isotropic grain growth, single phase
     out.append(("gg2d_32", StepConfig(32, 32, 0), 8, 11, None, None, 40, [0, 1, 2, 5, 20, 40], 1.0))
@@ -71,6 +71,9 @@ def cases():
                      dg_mode=np.array([[0, 1], [1, 0]], np.int32), dg_value=np.array([[0.0, 2e5], [-2e5, 0.0]]),
                      dg_average=True, dg_weights=2)
     out.append(("avg_counter2d_24", cnt, 30, 19, (np.arange(30) % 2).astype(np.int32), None, 30, [0, 1, 5, 30], 1.0))
+    # the averaged three-phase driving force set only in every third step (the driver skips
+    # UserDrivingForce::SetDrivingForce in the others)
+    out.append(("dg_every3_2d_32", three, 12, 23, phase3, None, 12, [0, 1, 3, 7, 12], 0.9, 3))
     return out
 
 
@@ -114,14 +117,15 @@ def main_io(only=()):
 
 
 def main(only=()):
-    for name, cfg, ng, seed, phase, quat, nsteps, dumps, dtf in cases():
+    for name, cfg, ng, seed, phase, quat, nsteps, dumps, dtf, *rest in cases():
         if only and name not in only:
             continue
+        dg_every = rest[0] if rest else 1
         lists = voronoi_grains(cfg.nx, cfg.ny, cfg.nz, ng, seed=seed)
         phase = np.zeros(ng, np.int32) if phase is None else phase
         dt = cfg.max_time_step() * dtf
         with tempfile.TemporaryDirectory() as td:
-            timing, ref = O.run_reference(td, cfg, lists, phase, quat, nsteps, dt, dumps)
+            timing, ref = O.run_reference(td, cfg, lists, phase, quat, nsteps, dt, dumps, energy=True, dg_every=dg_every)
         rec = dict(
             nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, dx=cfg.dx, iwidth=cfg.iwidth, nphases=cfg.nphases,
             sigma=np.asarray(cfg.sigma, np.float64), mu=np.asarray(cfg.mu, np.float64),
@@ -141,6 +145,10 @@ def main(only=()):
             # step's IP.Set; plausibility warnings of NormalizeIncrements over the run
             max_dt_init=float(timing["max_dt_init"]), max_dt=np.asarray(timing["max_dt"], np.float64),
             limiter_warnings=int(timing["limiter_warnings"]),
+            # DoubleObstacle::AverageEnergyDensity after every step, and after one more IP.Set on
+            # the final state; the step period of UserDrivingForce::SetDrivingForce
+            energy=np.asarray(timing["energy"], np.float64), energy_ip_current=float(timing["energy_ip_current"]),
+            dg_every=int(dg_every),
         )
         for s in dumps:
             st, vol, vr = ref[s]

@@ -23,7 +23,12 @@ def test_oracle_matches_reference_bitwise(name):
         if s > 0:
             # maxima of energy and mobility the step's IP.Set saw (CUBIC: orientation dependent)
             assert orc.max_time_step() == case.max_dt[s - 1], f"{name} step {s}: ReportMaximumTimeStep differs"
+            # DoubleObstacle::AverageEnergyDensity with that IP.Set's energies (one thread: the
+            # reference's summation order)
+            assert orc.interface_energy("previous") / case.ncells == case.energy[s - 1], f"{name} step {s}: energy"
     assert orc.limiter_warnings() == case.limiter_warnings
+    assert orc.interface_energy("current") / case.ncells == case.energy_ip_current
+    assert orc.interface_energy("none") == 0.0
 
 
 def test_max_time_step_closed_forms():
