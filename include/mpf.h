@@ -156,6 +156,32 @@ mpf_status mpf_get_grain_records(mpf_ctx* ctx, double* maxvolume, double* refvol
  * before its first IP.Set. */
 mpf_status mpf_max_time_step(mpf_ctx* ctx, double* dt_max);
 
+/* State the interface energies of InterfaceProperties::Properties come from (what the last
+ * IP.Set evaluated): no IP.Set yet, the current state, or the state the last step started from
+ * (the usual IP.Set -> step -> energy order). */
+typedef enum mpf_ip_state {
+    MPF_IP_NONE = 0,
+    MPF_IP_CURRENT = 1,
+    MPF_IP_PREVIOUS = 2
+} mpf_ip_state;
+
+/* DoubleObstacle::PointEnergy (src/DoubleObstacle.cpp:903-923) summed over every cell of the
+ * current state: DoubleObstacle::AverageEnergyDensity (:989-1005) is sum / cells, Energy
+ * (:953-967) sum * cell volume. Per wide-interface cell and pair of its neighbour-merged list,
+ * 4 E (phi_a phi_b - Eta^2/Pi^2 grad_a . grad_b) / Eta, E the pair's energy from the IP.Set state
+ * named by ip_state (0 for a pair absent there). Summed on the device in a fixed order (the same
+ * from run to run; the reference's sequential sum differs in rounding only). MPF_IP_PREVIOUS needs
+ * a step since the last upload / restore / finalize. */
+mpf_status mpf_interface_energy(mpf_ctx* ctx, int32_t ip_state, double* sum);
+
+/* Terms of the following steps (default: as created). driving_force 0 leaves the constant
+ * driving force out: the reference applies it only in steps whose driver calls
+ * UserDrivingForce::SetDrivingForce (src/UserDrivingForce.cpp:152-175), DrivingForce::Average
+ * (src/DrivingForce.cpp:242-256) and DrivingForce::MergePhaseFieldIncrements (:480-495); 1 needs
+ * a context created with a driving force. average 0 takes the raw value (DrivingForce::SkipAverage,
+ * :258-274) in a context created with averaging; without averaging it has no effect. */
+mpf_status mpf_set_step_terms(mpf_ctx* ctx, int32_t driving_force, int32_t average);
+
 /* Number of limiter plausibility warnings (PhaseField.cpp:1536-1561) since creation. */
 mpf_status mpf_limiter_warnings(mpf_ctx* ctx, int64_t* count);
 
@@ -167,9 +193,11 @@ mpf_status mpf_overflow_cell(mpf_ctx* ctx, int64_t* cell);
  * and the HIP stream (hipStream_t) the context runs on. Any output pointer may be NULL. */
 mpf_status mpf_device_state(mpf_ctx* ctx, void** count, void** index, void** value, void** stream);
 
-/* Kernel timing of the last mpf_step call: average duration in ms of the fused step kernel and
- * of the grain-volume kernel, measured with HIP events on the context's stream. */
-mpf_status mpf_last_timing(mpf_ctx* ctx, double* step_kernel_ms, double* grain_kernel_ms);
+/* Kernel timing of the last mpf_step call (timing on), average per step in ms, measured with HIP
+ * events on the context's stream: the main step kernel (with the driving-force averaging
+ * kernels when they run), and the rest of the step - the deferred-cell kernels and the grain
+ * kernel - in `rest_ms`. */
+mpf_status mpf_last_timing(mpf_ctx* ctx, double* step_kernel_ms, double* rest_ms);
 
 /* Enables/disables per-kernel HIP event timing in mpf_step (off by default). */
 mpf_status mpf_enable_timing(mpf_ctx* ctx, int32_t on);

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cfloat>
+#include <cmath>
 #include <cstdint>
 #include <fstream>
 #include <map>
@@ -121,6 +122,11 @@ struct GridParameters {
     long TotalNumberOfCells() const
     {
         return static_cast<long>(Nx ? Nx : 1) * (Ny ? Ny : 1) * (Nz ? Nz : 1);
+    }
+    // volume in 3D, area in 2D, length in 1D of one cell
+    double CellVolume(bool ignore_active_dimensions = false) const
+    {
+        return std::pow(dx, ignore_active_dimensions ? 3 : Active());
     }
 };
 
@@ -372,13 +378,25 @@ class PhaseField {
         check_bc(BC);
         if (!finalize || !clear) throw std::logic_error("MergeIncrements without finalize/clear is not supported");
         if (stage_ != 3 || dt != dt_) throw std::logic_error("MergeIncrements must follow NormalizeIncrements with the same dt");
+        // the constant driving force acts in this step only if the driver set, averaged and
+        // merged it (DrivingForce::MergePhaseFieldIncrements after UserDrivingForce::SetDrivingForce
+        // and DrivingForce::Average)
+        if (driving_force_)
+            check(mpf_set_step_terms(ctx_, dg_step_ != 0, dg_step_ == 2), ctx_, "PhaseField::MergeIncrements");
         check(mpf_step(ctx_, dt, 1), ctx_, "PhaseField::MergeIncrements");
         stage_ = 0;
+        dg_step_ = 0;
+        ++steps_;
     }
-    // Runs n steps of the recorded sequence at once (one synchronisation)
+    // Runs n steps of the recorded sequence at once (one synchronisation): the calls of the last
+    // step, its driving-force terms included, IP.Set in each
     void Advance(double dt, int nsteps)
     {
         check(mpf_step(ctx_, dt, nsteps), ctx_, "PhaseField::Advance");
+        if (nsteps > 0) {
+            ip_step_ = steps_ + nsteps - 1;
+            steps_ += nsteps;
+        }
     }
     // GrainsProperties Volume / VolumeRatio
     void GrainVolumes(std::vector<double>& volume, std::vector<double>& ratio) const
@@ -417,7 +435,10 @@ class PhaseField {
     mpf_ctx* ctx_ = nullptr;
     int stage_ = 0;      // 0 idle, 1 IP.Set done, 2 increments done, 3 normalized
     double dt_ = 0.0;
-    bool driving_force_ = false;
+    bool driving_force_ = false;  // the input file gives a constant driving force (device tables)
+    int dg_step_ = 0;    // driving force merged into this step: 0 none, 1 raw (SkipAverage), 2 averaged
+    long steps_ = 0;     // steps run on the device
+    long ip_step_ = -1;  // steps_ at the last IP.Set on this phase field (-1: none)
 };
 
 // ------------------------------------------------------------------------------------------
@@ -461,6 +482,7 @@ class InterfaceProperties {
         Phase.check_bc(BC);
         Phase.ensure_context();
         Phase.stage_ = 1;
+        Phase.ip_step_ = Phase.steps_;
         phase_ = &Phase;
     }
     // InterfaceProperties::ReportMaximumTimeStep (src/InterfaceProperties.cpp:523-556): from the
@@ -484,6 +506,7 @@ class InterfaceProperties {
 
  private:
     friend class PhaseField;
+    friend class DoubleObstacle;
     double dx_;
     int active_;
     const PhaseField* phase_ = nullptr;  // the phase field of the last Set
@@ -512,7 +535,9 @@ class UserDrivingForce {
                 }
             }
     }
-    void SetDrivingForce(PhaseField& Phase, DrivingForce&) { (void)Phase; }
+    // UserDrivingForce::SetDrivingForce (src/UserDrivingForce.cpp:152-175): the VALUE pairs' raw
+    // driving force into dG, which DrivingForce::Average and MergePhaseFieldIncrements then apply
+    void SetDrivingForce(PhaseField& Phase, DrivingForce& dG);
     size_t Nphases;
     std::vector<int32_t> Mode;
     std::vector<double> Value;
@@ -532,17 +557,48 @@ class DrivingForce {
         WeightsMode = w == "RANGE" ? MPF_DG_WEIGHTS_RANGE : (w == "COUNTER" ? MPF_DG_WEIGHTS_COUNTER : MPF_DG_WEIGHTS_PHASEFIELDS);
         Limit = in.getd("DrivingForce", "Limit_0_0", 0.95);
     }
-    // The averaging and the merge run inside the fused step of PhaseField::MergeIncrements.
-    void Clear() {}
-    void Average(const PhaseField&, const BoundaryConditions&) {}
-    void MergePhaseFieldIncrements(PhaseField&, InterfaceProperties&) {}
+    // The averaging and the merge run inside the fused step of PhaseField::MergeIncrements; these
+    // calls record which of them the driver asked for in this step.
+    // DrivingForce::Clear (src/DrivingForce.cpp:176-183)
+    void Clear() { raw_ = averaged_ = false; }
+    // DrivingForce::Average (src/DrivingForce.cpp:242-256): CollectAverage / DistributeAverage,
+    // or SkipAverage when Averaging is off; without it the merged force weighs 0
+    void Average(const PhaseField&, const BoundaryConditions&) { averaged_ = true; }
+    // DrivingForce::MergePhaseFieldIncrements (src/DrivingForce.cpp:480-495): after the
+    // double-obstacle increments, before PhaseField::NormalizeIncrements
+    void MergePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties&)
+    {
+        if (Phase.stage_ != 2)
+            throw std::logic_error("DrivingForce::MergePhaseFieldIncrements must follow CalculatePhaseFieldIncrements "
+                                   "and precede NormalizeIncrements");
+        Phase.dg_step_ = raw_ && averaged_ ? (Averaging ? 2 : 1) : 0;
+    }
     size_t Nphases;
     bool Averaging = true;
     int Range = 5;
     double PhiThreshold = 1.0 / 3.0;
     int WeightsMode = MPF_DG_WEIGHTS_PHASEFIELDS;
     double Limit = 0.95;
+
+ private:
+    friend class UserDrivingForce;
+    bool raw_ = false;       // a raw driving force was set since the last Clear
+    bool averaged_ = false;  // Average ran since the last Clear
 };
+
+inline void UserDrivingForce::SetDrivingForce(PhaseField& Phase, DrivingForce& dG)
+{
+    bool any = false;
+    for (int32_t m : Mode) any = any || m == 1;
+    if (!any) return;
+    Phase.ensure_context();
+    if (!Phase.driving_force_)
+        throw std::logic_error("UserDrivingForce::SetDrivingForce: the phase field's input file gives no driving force");
+    // the reference adds the raw values of a second call up (NodeDF::add_raw); that sum is not
+    // modelled here
+    if (dG.raw_) throw std::logic_error("UserDrivingForce::SetDrivingForce twice without DrivingForce::Clear");
+    dG.raw_ = true;
+}
 
 // ------------------------------------------------------------------------------------------
 // DoubleObstacle (include/DoubleObstacle.h)
@@ -556,10 +612,26 @@ class DoubleObstacle {
         if (Phase.stage_ != 1) throw std::logic_error("CalculatePhaseFieldIncrements before IP.Set");
         Phase.stage_ = 2;
     }
+    // The overload with the driving force computes the same increments: the force acts through
+    // DrivingForce::MergePhaseFieldIncrements (src/DoubleObstacle.cpp:478-500, FullAnisotropy off)
     void CalculatePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties& IP, DrivingForce&)
     {
         CalculatePhaseFieldIncrements(Phase, IP);
     }
+    // DoubleObstacle::AverageEnergyDensity (src/DoubleObstacle.cpp:989-1005): PointEnergy summed
+    // over the grid / number of cells, with the interface energies of IP's last Set on Phase
+    double AverageEnergyDensity(const PhaseField& Phase, const InterfaceProperties& IP) const
+    {
+        return energy_sum(Phase, IP) / static_cast<double>(Phase.Grid.TotalNumberOfCells());
+    }
+    // DoubleObstacle::Energy (src/DoubleObstacle.cpp:953-967): the sum times the cell volume
+    double Energy(const PhaseField& Phase, const InterfaceProperties& IP) const
+    {
+        return energy_sum(Phase, IP) * Phase.Grid.CellVolume();
+    }
+
+ private:
+    static double energy_sum(const PhaseField& Phase, const InterfaceProperties& IP);
 };
 
 // Builds the device context from the input file the first time the state is needed.
@@ -619,6 +691,21 @@ inline void PhaseField::ensure_context()
               ctx_, "PhaseField::AddGrainInfo");
         grains_dirty_ = false;
     }
+}
+
+inline double DoubleObstacle::energy_sum(const PhaseField& Phase, const InterfaceProperties& IP)
+{
+    // Properties hold what IP's last Set on this phase field evaluated: nothing before one (every
+    // energy 0), the current state when no step ran since, else the state the last step started
+    // from; an older Set is not held by the device state
+    if (IP.phase_ != &Phase || Phase.ip_step_ < 0 || !Phase.ctx_) return 0.0;
+    int32_t ip = MPF_IP_NONE;
+    if (Phase.ip_step_ == Phase.steps_) ip = MPF_IP_CURRENT;
+    else if (Phase.ip_step_ == Phase.steps_ - 1) ip = MPF_IP_PREVIOUS;
+    else throw std::logic_error("DoubleObstacle energy: IP.Set ran before the last step's start");
+    double sum = 0.0;
+    check(mpf_interface_energy(Phase.ctx_, ip, &sum), Phase.ctx_, "DoubleObstacle::AverageEnergyDensity");
+    return sum;
 }
 
 }  // namespace openphase_amd

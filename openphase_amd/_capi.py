@@ -23,13 +23,15 @@ MPF_FP64, MPF_FP32 = 0, 1
 MPF_TJ_MODEL, MPF_TJ_VALUE, MPF_TJ_NONE = 0, 1, 2
 MPF_ENERGY_ISO, MPF_ENERGY_CUBIC = 0, 1
 MPF_DG_WEIGHTS_PHASEFIELDS, MPF_DG_WEIGHTS_RANGE, MPF_DG_WEIGHTS_COUNTER = 0, 1, 2
+MPF_IP_NONE, MPF_IP_CURRENT, MPF_IP_PREVIOUS = 0, 1, 2
 
 # Every symbol the header declares (checked by the CPU tests without calling into the GPU).
 EXPORTED = [
     "mpf_create", "mpf_set_grains", "mpf_upload", "mpf_finalize_initialization", "mpf_step",
     "mpf_step_async", "mpf_synchronize", "mpf_total_entries", "mpf_download", "mpf_get_grains",
     "mpf_get_grain_records",
-    "mpf_max_time_step", "mpf_limiter_warnings", "mpf_overflow_cell", "mpf_device_state",
+    "mpf_max_time_step", "mpf_interface_energy", "mpf_set_step_terms", "mpf_limiter_warnings",
+    "mpf_overflow_cell", "mpf_device_state",
     "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
     "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_autotune",
     "mpf_set_launch_shape", "mpf_set_tile_shape", "mpf_get_tile_shape", "mpf_ensemble_create", "mpf_ensemble_step", "mpf_ensemble_destroy",
@@ -94,6 +96,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_get_grains": ([vp, f64p, f64p], C.c_int),
         "mpf_get_grain_records": ([vp, f64p, f64p, C.POINTER(C.c_uint8), C.POINTER(C.c_int32)], C.c_int),
         "mpf_max_time_step": ([vp, f64p], C.c_int),
+        "mpf_interface_energy": ([vp, C.c_int32, f64p], C.c_int),
+        "mpf_set_step_terms": ([vp, C.c_int32, C.c_int32], C.c_int),
         "mpf_limiter_warnings": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "mpf_overflow_cell": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "mpf_device_state": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)], C.c_int),

@@ -45,6 +45,7 @@ def build_hip() -> Path:
     # (source, object name, extra defines): the per-cell step kernels build as four variants
     units = [(CSRC / "mpf_kernels.hip", "mpf_kernels", []), (CSRC / "mpf_tile.hip", "mpf_tile", []),
              (CSRC / "mpf_adjoint.hip", "mpf_adjoint", []), (CSRC / "mpf_dgavg.hip", "mpf_dgavg", []),
+             (CSRC / "mpf_energy.hip", "mpf_energy", []),
              (CSRC / "mpf_capi.cpp", "mpf_capi", [])]
     for fp32 in (0, 1):
         for full in (0, 1):
@@ -68,12 +69,13 @@ def build_hip() -> Path:
 
 
 def build_examples() -> None:
-    """The C++ drop-in example (OpenPhase-style classes over the C ABI), linked to libmpf_hip.so."""
+    """The C++ drop-in examples (OpenPhase-style classes over the C ABI), linked to libmpf_hip.so."""
     cxx = shutil.which("g++") or "g++"
-    src = PKG / "examples" / "normal_gg.cpp"
-    out = PKG / "examples" / "normal_gg"
-    _run([cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(src), f"-L{PKG}", "-lmpf_hip",
-          "-Wl,-rpath,$ORIGIN/..", "-o", str(out)])
+    for name in ("normal_gg", "driving_force"):
+        src = PKG / "examples" / f"{name}.cpp"
+        out = PKG / "examples" / name
+        _run([cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(src), f"-L{PKG}", "-lmpf_hip",
+              "-Wl,-rpath,$ORIGIN/..", "-o", str(out)])
 
 
 def build_oracle() -> None:

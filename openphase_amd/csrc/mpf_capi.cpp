@@ -68,6 +68,9 @@ struct mpf_ctx {
     double* d_avg = nullptr;
     int32_t* d_avg_off = nullptr;
     double* d_avg_off_w = nullptr;
+    // per-step terms (mpf_set_step_terms): the driving force acts / is averaged in the next steps
+    bool dg_on = true, avg_on = true;
+    int32_t* d_dgmode_off = nullptr;  // all-zero mode table: no pair takes the driving force
 
     hipStream_t stream = nullptr;
     uint8_t* cnt[2] = {nullptr, nullptr};
@@ -111,6 +114,11 @@ struct mpf_ctx {
     unsigned long long* d_stats = nullptr;
     unsigned long long* d_obs = nullptr;  // IP.Set maxima: this step's (2 words), the last step's (2)
     bool stepped = false;                 // a step ran since creation (IP.Set maxima exist)
+    // the other buffer still holds the state the last step started from (interface energies of
+    // that step's IP.Set)
+    bool prev_valid = false;
+    double* d_energy = nullptr;           // per-block partial sums of the interface energy, then the sum
+    size_t energy_cap = 0;
 
     int xcd_order = 1;               // launch shape, see StepArgs
     int deferred_grid = mpf::kWideGrid;
@@ -173,6 +181,13 @@ struct mpf_ensemble {
 
 namespace {
 
+// The driving force acts in the next step: the context has one and mpf_set_step_terms left it on.
+bool dg_active(const mpf_ctx* c) { return c->dg && c->dg_on; }
+// DrivingForce::Average averages it over the interface in the next step.
+bool avg_active(const mpf_ctx* c) { return c->dg_avg && c->dg_on && c->avg_on; }
+// Kernel variant of the next step: orientation-dependent energies and/or an acting driving force.
+bool full_variant(const mpf_ctx* c) { return c->grad || dg_active(c); }
+
 void free_grains(mpf_ctx* c)
 {
     dfree(c->g_phase); dfree(c->g_vr); dfree(c->g_rot); dfree(c->g_solid);
@@ -200,8 +215,9 @@ mpf::StepArgs step_args(const mpf_ctx* c, double dt, int src)
     a.sigma0 = c->sigma[0];
     a.mu0 = c->mu[0];
     a.sigma = c->d_sigma; a.mu = c->d_mu; a.eps = c->d_eps; a.max_energy = c->d_maxe;
-    a.emodel = c->d_emodel; a.dg_mode = c->d_dgmode; a.dg_value = c->d_dgval; a.dg_allowed = c->d_dgallowed;
-    a.dg_avg = c->dg_avg ? c->d_avg : nullptr;
+    a.emodel = c->d_emodel; a.dg_value = c->d_dgval; a.dg_allowed = c->d_dgallowed;
+    a.dg_mode = dg_active(c) ? c->d_dgmode : c->d_dgmode_off;
+    a.dg_avg = avg_active(c) ? c->d_avg : nullptr;
     a.gphase = c->g_phase; a.gvr = c->g_vr; a.grot = c->g_rot; a.gsolid = c->g_solid;
     a.cnt_in = c->cnt[src]; a.id_in = c->id[src]; a.val_in = c->val[src];
     a.cnt_out = c->cnt[1 - src]; a.id_out = c->id[1 - src]; a.val_out = c->val[1 - src];
@@ -273,7 +289,7 @@ unsigned tile_blocks(const mpf_ctx* c)
 // per-cell kernel.
 hipError_t launch_main(const mpf_ctx* c, const mpf::StepArgs& a, hipStream_t stream)
 {
-    const bool full = c->grad || c->dg;
+    const bool full = full_variant(c);
     if (c->tile_x > 0) return static_cast<hipError_t>(mpf::launch_tile(a, tile_blocks(c), c->fp32, full, c->uniform, stream));
     return static_cast<hipError_t>(mpf::launch_step(a, nullptr, nullptr, 1, step_blocks(c), c->fp32, full, c->uniform, stream));
 }
@@ -325,19 +341,17 @@ mpf::AvgArgs avg_args(const mpf_ctx* c, const mpf::StepArgs& s)
 
 mpf_status launch_step_part(mpf_ctx* c, hipStream_t stream)
 {
-    const bool full = c->grad || c->dg;
-    if (c->dg_avg) {
+    if (avg_active(c)) {
         mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_dg_average(avg_args(c, c->h_args[c->cur]), c->fp32, stream)),
                                "driving-force averaging kernels");
         if (st) return st;
     }
-    (void)full;
     return c->hip(launch_main(c, c->h_args[c->cur], stream), "step kernel");
 }
 
 mpf_status launch_rest_part(mpf_ctx* c, hipStream_t stream)
 {
-    const bool full = c->grad || c->dg;
+    const bool full = full_variant(c);
     mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(c->d_args + c->cur, 1, c->deferred_grid, c->fp32,
                                                                         full, stream)), "deferred-cell kernels");
     if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_grains(c->d_gargs, c->d_gfirst, 1, grain_blocks(c), stream)),
@@ -520,6 +534,7 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         c->hip(dalloc(&c->d_eps, np2), "tables") || c->hip(dalloc(&c->d_maxe, np2), "tables") ||
         c->hip(dalloc(&c->d_emodel, np2), "tables") || c->hip(dalloc(&c->d_dgmode, np2), "tables") ||
         c->hip(dalloc(&c->d_dgval, np2), "tables") || c->hip(dalloc(&c->d_dgallowed, np2), "tables") ||
+        c->hip(dalloc(&c->d_dgmode_off, np2), "tables") ||
         c->hip(dalloc(&c->d_status, mpf::kStatusWords), "status") ||
         c->hip(dalloc(&c->d_first_overflow, 1), "status") || c->hip(dalloc(&c->d_warnings, 1), "status") ||
         c->hip(dalloc(&c->d_work_list, 4 * N), "work list") || c->hip(dalloc(&c->d_work_count, 4), "work list") ||
@@ -539,6 +554,8 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     h2d(e, c->d_maxe, c->maxe.data(), np2);
     h2d(e, c->d_emodel, c->emodel.data(), np2);
     h2d(e, c->d_dgmode, c->dgmode.data(), np2);
+    const std::vector<int32_t> no_mode(np2, 0);
+    h2d(e, c->d_dgmode_off, no_mode.data(), np2);
     h2d(e, c->d_dgval, c->dgval.data(), np2);
     h2d(e, c->d_dgallowed, c->dgallowed.data(), np2);
     if (c->dg_avg) {
@@ -625,6 +642,7 @@ mpf_status mpf_set_grains(mpf_ctx* c, int32_t ngrains, const int32_t* phase, con
     // an uploaded state was checked against the previous grain count: with fewer grains its
     // indices may be out of range, so it has to be uploaded again
     if (ngrains < c->ngrains) c->have_state = false;
+    c->prev_valid = false;
     c->ngrains = ngrains;
     c->have_grains = true;
     c->initialized = false;
@@ -667,6 +685,7 @@ mpf_status mpf_upload(mpf_ctx* c, const int32_t* count, const int32_t* index, co
     if (s) return s;
     c->have_state = true;
     c->initialized = false;
+    c->prev_valid = false;
     return MPF_OK;
 }
 
@@ -683,6 +702,7 @@ mpf_status mpf_finalize_initialization(mpf_ctx* c)
                "grain kernel");
     if (s) return s;
     c->cur = 1 - c->cur;
+    c->prev_valid = false;
     s = read_status(c);
     if (s) return s;
     c->initialized = true;
@@ -717,6 +737,7 @@ mpf_status mpf_step_async(mpf_ctx* c, double dt, int32_t nsteps)
         if (st) return st;
         c->cur = 1 - c->cur;
         c->stepped = true;
+        c->prev_valid = true;
     }
     if (timing) {
         if (mpf_status st = c->hip(hipEventSynchronize(c->events[3 * (nsteps - 1) + 2]), "hipEventSynchronize")) return st;
@@ -866,6 +887,54 @@ mpf_status mpf_max_time_step(mpf_ctx* c, double* dt_max)
     return MPF_OK;
 }
 
+mpf_status mpf_interface_energy(mpf_ctx* c, int32_t ip_state, double* sum)
+{
+    if (!c || !sum) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
+    if (ip_state < MPF_IP_NONE || ip_state > MPF_IP_PREVIOUS) return c->fail(MPF_ERR_ARG, "unknown ip_state");
+    if (ip_state == MPF_IP_PREVIOUS && !c->prev_valid)
+        return c->fail(MPF_ERR_STATE, "the state the last IP.Set saw is held only after a step (none since the last "
+                                      "upload, restore or finalize)");
+    const size_t nb = static_cast<size_t>(mpf::energy_blocks(c->ncells));
+    if (c->energy_cap < nb + 1) {
+        dfree(c->d_energy);
+        c->energy_cap = 0;
+        if (c->hip(dalloc(&c->d_energy, nb + 1), "energy partial sums")) return MPF_ERR_NOMEM;
+        c->energy_cap = nb + 1;
+    }
+    mpf::EnergyArgs a{};
+    a.cur = step_args(c, 1.0, c->cur);
+    if (ip_state != MPF_IP_NONE) {
+        const int ip = ip_state == MPF_IP_CURRENT ? c->cur : 1 - c->cur;
+        a.ip_cnt = c->cnt[ip];
+        a.ip_id = c->id[ip];
+        a.ip_val = c->val[ip];
+    }
+    const double eta = c->grid.iwidth * c->grid.dx;
+    a.eta = eta;
+    a.pref = eta * eta / (kPi * kPi);
+    a.partial = c->d_energy;
+    a.status = c->d_status + 3;
+    const int32_t zero = 0;
+    int32_t flag = 0;
+    mpf_status st = c->hip(hipMemcpyAsync(a.status, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream), "status reset");
+    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_energy(a, c->fp32, c->d_energy + nb, c->stream)), "energy kernels");
+    if (!st) st = c->hip(hipMemcpyAsync(sum, c->d_energy + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream), "energy readback");
+    if (!st) st = c->hip(hipMemcpyAsync(&flag, a.status, sizeof(flag), hipMemcpyDeviceToHost, c->stream), "status readback");
+    if (!st) st = c->hip(hipStreamSynchronize(c->stream), "stream synchronize");
+    if (!st && flag) st = c->fail(MPF_SLOT_OVERFLOW, "a cell's merged list exceeds 16 entries");
+    return st;
+}
+
+mpf_status mpf_set_step_terms(mpf_ctx* c, int32_t driving_force, int32_t average)
+{
+    if (!c) return MPF_ERR_ARG;
+    if (driving_force && !c->dg) return c->fail(MPF_ERR_STATE, "the context was created without a driving force");
+    c->dg_on = driving_force != 0;
+    c->avg_on = average != 0;
+    return MPF_OK;
+}
+
 mpf_status mpf_limiter_warnings(mpf_ctx* c, int64_t* count)
 {
     if (!c || !count) return MPF_ERR_ARG;
@@ -965,7 +1034,8 @@ void mpf_destroy(mpf_ctx* c)
     dfree(c->a_lam_own); dfree(c->a_cnt); dfree(c->a_id); dfree(c->a_lap); dfree(c->a_grad);
     dfree(c->d_args); dfree(c->d_first); dfree(c->d_gargs); dfree(c->d_gfirst);
     dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
-    dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed);
+    dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed); dfree(c->d_dgmode_off);
+    dfree(c->d_energy);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
     dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats); dfree(c->d_obs);
     dfree(c->d_avg_tmp); dfree(c->d_avg); dfree(c->d_avg_off); dfree(c->d_avg_off_w);
@@ -1073,6 +1143,7 @@ mpf_status mpf_restore_state(mpf_ctx* c, const void* src)
     auto cp = [&](void* dstp, size_t off, size_t n) {
         if (e == hipSuccess) e = hipMemcpyAsync(dstp, s + off, n, hipMemcpyDeviceToDevice, c->stream);
     };
+    c->prev_valid = false;
     cp(c->cnt[c->cur], L.cnt, N);
     cp(c->id[c->cur], L.id, S * N * sizeof(uint16_t));
     cp(c->val[c->cur], L.val, S * N * c->vsize());
@@ -1090,7 +1161,7 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
     if (!c || !lam_next || !lam_prev || !grad) return MPF_ERR_ARG;
     if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
     if (c->fp32) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint runs on FP64 states");
-    if (c->dg_avg) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint takes the driving force without averaging");
+    if (avg_active(c)) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint takes the driving force without averaging");
     if (c->nphases > 4) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint handles at most 4 phases");
     const size_t N = static_cast<size_t>(c->ncells);
     constexpr size_t KA = mpf::kAdjSlots;
@@ -1114,7 +1185,7 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
     a.dg_limit = c->dg_limit;
     a.regularization = c->regularization;
     a.grad_energy = c->grad ? 1 : 0;
-    a.driving_force = c->dg ? 1 : 0;
+    a.driving_force = dg_active(c) ? 1 : 0;
     const int32_t zeros[mpf::kStatusWords] = {0, 0, 0, 0};
     mpf_status st = c->hip(hipMemcpyAsync(c->d_status, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream), "status reset");
     if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_adjoint(a, c->stream)), "adjoint kernels");
@@ -1326,7 +1397,7 @@ mpf_status capture(mpf_ensemble* e)
             const unsigned blocks = g.h_first[n], gblocks = g.h_gfirst[n];
             for (int k = 0; k < n && rc == 0; ++k) {
                 const mpf_ctx* c = e->members[g.members[k]];
-                if (c->dg_avg) rc = mpf::launch_dg_average(avg_args(c, half == 0 ? g.h_a1[k] : g.h_a2[k]), g.fp32, e->stream);
+                if ((half == 0 ? g.h_a1[k] : g.h_a2[k]).dg_avg) rc = mpf::launch_dg_average(avg_args(c, half == 0 ? g.h_a1[k] : g.h_a2[k]), g.fp32, e->stream);
             }
             if (!rc) rc = mpf::launch_step(half == 0 ? g.h_a1[0] : g.h_a2[0], args, g.d_first, n, blocks, g.fp32, g.full, g.uni,
                                   e->stream);
@@ -1366,6 +1437,8 @@ mpf_status mpf_ensemble_create(mpf_ctx* const* members, int32_t n, mpf_ensemble*
     e->members.assign(members, members + n);
     for (int32_t i = 0; i < n; ++i) {
         const mpf_ctx* c = members[i];
+        // a member with a driving force runs the full variant whether mpf_set_step_terms switched
+        // it on or off (off: an all-zero mode table), so the grouping holds across switches
         const bool full = c->grad || c->dg;
         mpf_group* g = nullptr;
         for (auto& q : e->groups)
@@ -1409,7 +1482,7 @@ mpf_status mpf_ensemble_step(mpf_ensemble* e, const double* dt, int32_t nsteps)
         for (size_t i = 0; i < n && !st; ++i) st = enqueue_step(e->members[i], dt[i], e->members[i]->stream);
     for (size_t i = 0; i < n && !st; ++i) st = read_status(e->members[i]);
     if (!st && nsteps > 0)
-        for (auto* m : e->members) m->stepped = true;
+        for (auto* m : e->members) m->stepped = m->prev_valid = true;
     return st;
 }
 
@@ -1523,7 +1596,7 @@ mpf_status mpf_step_into(mpf_ctx* c, double dt, const void* count_in, const void
         return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: " + std::to_string(c->ncells) + " cells exceed its scratch limit of " +
                                                 std::to_string(kScratchCells) + " cells per call (use mpf_step)");
     if (c->ngrains > kScratchGrains) return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: too many grains");
-    if (c->dg_avg)
+    if (avg_active(c))
         return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: driving-force averaging keeps per-pair arrays of the context (use mpf_step)");
     unsigned char* base = nullptr;
     if (mpf_status s = scratch_address(c, &base)) return s;
@@ -1552,7 +1625,7 @@ mpf_status mpf_step_into(mpf_ctx* c, double dt, const void* count_in, const void
 
     hipLaunchKernelGGL(scratch_setup_kernel, dim3(64), dim3(256), 0, c->stream, base, c->ngrains, a, g, gblocks);
     if (mpf_status s = c->hip(hipGetLastError(), "scratch set-up kernel")) return s;
-    const bool full = c->grad || c->dg;
+    const bool full = full_variant(c);
     mpf_status st = c->hip(launch_main(c, a, c->stream), "step kernel");
     if (!st)
         st = c->hip(static_cast<hipError_t>(mpf::launch_deferred(reinterpret_cast<const mpf::StepArgs*>(base + kOffStepArgs), 1,

@@ -156,6 +156,22 @@ MPF_VARIANT_LAUNCHERS(v32_plain)
 MPF_VARIANT_LAUNCHERS(v32_full)
 #undef MPF_VARIANT_LAUNCHERS
 int launch_finalize_init(const StepArgs& a, bool fp32, void* stream);
+
+// Interface energy (mpf_energy.hip): `cur` the state summed over (cnt_in, id_in, val_in), grains
+// and tables; ip_* the state the last IP.Set saw (null: none, every energy 0).
+struct EnergyArgs {
+    StepArgs cur;
+    const uint8_t* ip_cnt;
+    const uint16_t* ip_id;
+    const void* ip_val;
+    double eta;                  // interface width
+    double pref;                 // Eta^2 / Pi^2
+    double* partial;             // energy_blocks(ncells) per-block sums
+    int32_t* status;             // set to 1 when a merged list exceeds the step's capacity
+};
+int64_t energy_blocks(int64_t ncells);
+// Writes the sum over all cells to `out` (device) after the per-block sums.
+int launch_energy(const EnergyArgs& a, bool fp32, double* out, void* stream);
 int launch_grains(const GrainArgs* args, const uint32_t* first, int ndom, unsigned blocks, void* stream);
 int launch_copy(const void* src, void* dst, int64_t bytes, void* stream);
 

@@ -177,6 +177,37 @@ __device__ __forceinline__ TC cubic_energy(const double* R, TC gx, TC gy, TC gz,
     return energy * (TC(1.0) + eps1 * (TC(1.5) - TC(2.5) * (q0 * q0 + q1 * q1 + q2 * q2)));
 }
 
+// Interface energy InterfaceProperties::SetSR gives the pair (alpha, beta) of a wide-interface
+// cell (src/InterfaceProperties.cpp:620-661), k = phase pair: the phase-pair energy (ISO), or the
+// cubic energy of each solid grain's interface normal weighted by phi (1 - phi), over the sum of
+// the weights, MaxEnergy when neither grain weighs (CUBIC). ga/gb: grains, va/vb: values,
+// da/db: gradients of the two entries.
+template <typename TC>
+__device__ __forceinline__ TC ip_pair_energy(const StepArgs& A, int k, uint32_t ga, uint32_t gb, TC va, TC vb,
+                                             const TC (&da)[3], const TC (&db)[3])
+{
+    if (A.emodel[k] == 0) return static_cast<TC>(A.sigma[k]);
+    const TC E = static_cast<TC>(A.sigma[k]), e1 = static_cast<TC>(A.eps[k]);
+    TC en = TC(0), wa = TC(0), wb = TC(0);
+    if (A.gsolid[ga]) {
+        wa = fabs(va * (TC(1.0) - va));
+        if (wa > TC(DBL_EPSILON))
+            en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(ga), da[0], da[1], da[2], E, e1) * wa;
+        else
+            wa = TC(0);
+    }
+    if (A.gsolid[gb]) {
+        wb = fabs(vb * (TC(1.0) - vb));
+        if (wb > TC(DBL_EPSILON))
+            en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(gb), db[0], db[1], db[2], E, e1) * wb;
+        else
+            wb = TC(0);
+    }
+    if (wa + wb > TC(DBL_EPSILON)) en /= wa + wb;
+    else en = static_cast<TC>(A.max_energy[k]);
+    return en;
+}
+
 enum CellResult { kCellBulk = 0, kCellWide = 1, kCellDeferred = 2, kCellOverflow = 3 };
 
 // Count byte of a stored list. FP64 states store the entry count. FP32 states store

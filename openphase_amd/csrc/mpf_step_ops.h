@@ -105,31 +105,8 @@ __device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, 
             for (int b = 0; b < C; ++b) {
                 if (b <= a || !(EXACT || b < ne)) continue;
                 const int k = ph[a] * NP + ph[b];
-                TC en = TC(0);
-                if (A.emodel[k] == 0) {
-                    en = static_cast<TC>(A.sigma[k]);
-                } else {
-                    const TC E = static_cast<TC>(A.sigma[k]), e1 = static_cast<TC>(A.eps[k]);
-                    TC wa = TC(0), wb = TC(0);
-                    if (A.gsolid[L.eid[a]]) {
-                        wa = fabs(L.ev(a) * (TC(1.0) - L.ev(a)));
-                        if (wa > TC(DBL_EPSILON))
-                            en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(L.eid[a]), L.gr[GRAD ? a : 0][0],
-                                                   L.gr[GRAD ? a : 0][1], L.gr[GRAD ? a : 0][2], E, e1) * wa;
-                        else
-                            wa = TC(0);
-                    }
-                    if (A.gsolid[L.eid[b]]) {
-                        wb = fabs(L.ev(b) * (TC(1.0) - L.ev(b)));
-                        if (wb > TC(DBL_EPSILON))
-                            en += cubic_energy<TC>(A.grot + 9 * static_cast<int64_t>(L.eid[b]), L.gr[GRAD ? b : 0][0],
-                                                   L.gr[GRAD ? b : 0][1], L.gr[GRAD ? b : 0][2], E, e1) * wb;
-                        else
-                            wb = TC(0);
-                    }
-                    if (wa + wb > TC(DBL_EPSILON)) en /= wa + wb;
-                    else en = static_cast<TC>(A.max_energy[k]);
-                }
+                const TC en = ip_pair_energy<TC>(A, k, L.eid[a], L.eid[b], L.ev(a), L.ev(b), L.gr[GRAD ? a : 0],
+                                                 L.gr[GRAD ? b : 0]);
                 sig[GRAD ? pair_index<C>(a, b) : 0] = en;
             }
         }

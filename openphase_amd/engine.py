@@ -212,6 +212,27 @@ class Engine:
         self._check(self.lib.mpf_max_time_step(self.h, C.byref(v)), "mpf_max_time_step")
         return v.value
 
+    IP_STATES = {"none": _capi.MPF_IP_NONE, "current": _capi.MPF_IP_CURRENT, "previous": _capi.MPF_IP_PREVIOUS}
+
+    def interface_energy(self, ip: str = "previous") -> float:
+        """Sum over cells of DoubleObstacle::PointEnergy with the interface energies of the last
+        IP.Set, evaluated on no state ("none"), the current state ("current") or the state the
+        last step started from ("previous"). / cells = AverageEnergyDensity."""
+        v = C.c_double()
+        self._check(self.lib.mpf_interface_energy(self.h, self.IP_STATES[ip], C.byref(v)), "mpf_interface_energy")
+        return v.value
+
+    def average_energy_density(self, ip: str = "previous") -> float:
+        """DoubleObstacle::AverageEnergyDensity (src/DoubleObstacle.cpp:989-1005)."""
+        return self.interface_energy(ip) / float(np.prod(self.cfg.shape))
+
+    def set_step_terms(self, driving_force: bool = True, average: bool = True) -> None:
+        """Driving force in the following steps or not (the reference's driver calls
+        UserDrivingForce::SetDrivingForce / DrivingForce::Average / MergePhaseFieldIncrements),
+        averaged or not (DrivingForce::SkipAverage) when the configuration averages."""
+        self._check(self.lib.mpf_set_step_terms(self.h, int(bool(driving_force)), int(bool(average))),
+                    "mpf_set_step_terms")
+
     def limiter_warnings(self) -> int:
         w = C.c_int64()
         self._check(self.lib.mpf_limiter_warnings(self.h, C.byref(w)), "mpf_limiter_warnings")

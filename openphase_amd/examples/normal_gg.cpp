@@ -1,15 +1,19 @@
 // Normal grain growth through the OpenPhase-style C++ interface (include/openphase_amd/OpenPhaseAMD.h):
-// the time loop of the reference's examples/NormalGG/NormalGG.cpp:74-85 (RunTimeControl loop,
+// the time loop of the reference's examples/NormalGG/NormalGG.cpp:74-107 (RunTimeControl loop,
 // IP.Set -> DO.CalculatePhaseFieldIncrements -> Phi.NormalizeIncrements -> Phi.MergeIncrements at
-// RTC.dt), on a microstructure read from a raw PhaseField .dat file instead of the reference's
-// Voronoi set-up, and without its console and VTK output inside the loop.
+// RTC.dt, DO.AverageEnergyDensity at the console-output steps), on a microstructure read from a raw
+// PhaseField .dat file instead of the reference's Voronoi set-up; the reference's timer, console
+// formatting, grain statistics and in-loop file output are left out.
 //
-// usage: normal_gg <input.opi> <state.dat> <ngrains> <nsteps> <out.dat> [outdir]
+// usage: normal_gg <input.opi> <state.dat> <ngrains | grains.bin> <nsteps> <out.dat> [outdir]
+// ngrains grains of phase 0, or the phases of grains.bin (uint32 n; n x { uint32 phase; double q[4] }).
 // The time step is the input file's @RunTimeControl $dt; <nsteps> steps run from its tStart.
 // With outdir, also the reference's output files of the last step there: PhaseField::WriteVTK
 // (VTK/PhaseField_<step>.vts) and GrainsProperties::Write (GrainsProperties.dat).
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <string>
 
 #include "openphase_amd/OpenPhaseAMD.h"
@@ -23,7 +27,25 @@ int main(int argc, char* argv[])
         return 2;
     }
     const std::string InputFile = argv[1];
-    const int ngrains = std::atoi(argv[3]);
+    std::ifstream gin(argv[3], std::ios::binary);
+    std::vector<uint32_t> phases;
+    if (gin) {
+        uint32_t ng = 0;
+        gin.read(reinterpret_cast<char*>(&ng), sizeof(ng));
+        for (uint32_t g = 0; g < ng && gin; ++g) {
+            uint32_t phase = 0;
+            double q[4];
+            gin.read(reinterpret_cast<char*>(&phase), sizeof(phase));
+            gin.read(reinterpret_cast<char*>(q), sizeof(q));
+            phases.push_back(phase);
+        }
+        if (!gin) {
+            std::fprintf(stderr, "could not read %s\n", argv[3]);
+            return 3;
+        }
+    } else {
+        phases.assign(static_cast<size_t>(std::atoi(argv[3])), 0u);
+    }
     const int nsteps = std::atoi(argv[4]);
     try {
         Settings OPSettings;
@@ -34,7 +56,7 @@ int main(int argc, char* argv[])
         DoubleObstacle DO(OPSettings, InputFile);
         InterfaceProperties IP(OPSettings, InputFile);
         BoundaryConditions BC(OPSettings, InputFile);
-        for (int g = 0; g < ngrains; ++g) Phi.AddGrainInfo(0);
+        for (uint32_t phase : phases) Phi.AddGrainInfo(phase);
         if (!Phi.Read(argv[2])) {
             std::fprintf(stderr, "could not read %s\n", argv[2]);
             return 3;
@@ -45,6 +67,8 @@ int main(int argc, char* argv[])
             DO.CalculatePhaseFieldIncrements(Phi, IP);
             Phi.NormalizeIncrements(BC, RTC.dt);
             Phi.MergeIncrements(BC, RTC.dt);
+            if (RTC.WriteToScreen())
+                std::printf("step %d interface energy density %.17g\n", RTC.tStep, DO.AverageEnergyDensity(Phi, IP));
         }
         const double dt = RTC.dt;
         if (!Phi.Write(argv[5])) return 4;

@@ -1,7 +1,10 @@
 """The OpenPhase-style C++ interface (include/openphase_amd/OpenPhaseAMD.h) driving the GPU step:
-the NormalGG time loop of the reference, unchanged, against the CPU oracle; its PhaseField::Write,
-WriteVTK and GrainsProperties::Write output against the output-format writers fed with the oracle's
-state (those writers match the reference's own files byte for byte)."""
+the NormalGG time loop of the reference (its step calls and AverageEnergyDensity) against the CPU
+oracle; its PhaseField::Write, WriteVTK and GrainsProperties::Write output against the
+output-format writers fed with the oracle's state (those writers match the reference's own files
+byte for byte); a driving-force loop in the reference's call order, with the driving force set in
+every step, every second step or never, and NormalGG's loop on an input file with a driving force
+it never applies."""
 import subprocess
 
 import numpy as np
@@ -16,6 +19,17 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 EXAMPLE = O.HERE.parent / "openphase_amd" / "examples" / "normal_gg"
+DG_EXAMPLE = O.HERE.parent / "openphase_amd" / "examples" / "driving_force"
+
+
+def _energies(stdout):
+    """{step: AverageEnergyDensity} from the examples' console lines."""
+    out = {}
+    for ln in stdout.splitlines():
+        if ln.startswith("step ") and "interface energy density" in ln:
+            parts = ln.split()
+            out[int(parts[1])] = float(parts[-1])
+    return out
 
 
 @pytest.mark.parametrize("shape,ngrains,steps", [((40, 40, 0), 10, 30), ((20, 20, 20), 8, 10)])
@@ -36,7 +50,13 @@ def test_normal_gg_example_matches_oracle(tmp_path, shape, ngrains, steps):
     orc.set_grains(np.zeros(ngrains, np.int32))
     orc.upload(lists)
     orc.finalize_initialization()
-    orc.step(cfg.max_time_step(), steps)
+    orc.keep_step_start(True)
+    # DoubleObstacle::AverageEnergyDensity at every console step (the input's interval is 1)
+    energies = _energies(res.stdout)
+    assert sorted(energies) == list(range(steps))
+    for t in range(steps):
+        orc.step(cfg.max_time_step(), 1)
+        assert energies[t] == pytest.approx(orc.interface_energy("previous") / lists.ncells, rel=1e-12)
     exp, flags = orc.download(with_flags=True)
     # IP.ReportMaximumTimeStep after the loop: from the maxima of the last IP.Set, as the oracle's
     reported = float(res.stdout.split("ReportMaximumTimeStep")[1].split()[0])
@@ -59,3 +79,87 @@ def test_normal_gg_example_matches_oracle(tmp_path, shape, ngrains, steps):
     assert_files_close(got, ref, 1e-12)
     if np.array_equal(full.value[keep], exp.value):
         assert got["dat"] == ref["dat"] and got["vtk"] == ref["vtk"]
+
+
+def _dg_case(tmp_path, steps):
+    cfg = StepConfig(40, 36, 0, nphases=3, sigma=np.full((3, 3), 0.24), mu=np.full((3, 3), 4e-9),
+                     dg_mode=np.ones((3, 3), np.int32) - np.eye(3, dtype=np.int32),
+                     dg_value=np.array([[0.0, 1e5, -5e4], [-1e5, 0.0, 2e5], [5e4, -2e5, 0.0]]), dg_average=True)
+    ng = 12
+    lists = voronoi_grains(40, 36, 0, ng, seed=27)
+    phase = (np.arange(ng) % 3).astype(np.int32)
+    dt = 0.9 * cfg.max_time_step()
+    O.write_opi(tmp_path / "in.opi", cfg, dt=dt, nsteps=steps)
+    rawio.write_phasefield(str(tmp_path / "in.dat"), lists)
+    rawio.write_grains(str(tmp_path / "grains.bin"), phase, None)
+    return cfg, lists, phase, dt
+
+
+def _oracle_run(cfg, lists, phase, dt, steps, every):
+    """Oracle lists and per-step energies with the driving force in the steps divisible by every
+    (0: never); steps count from the input's tStart, 0."""
+    orc = O.Oracle(cfg)
+    orc.set_grains(phase)
+    orc.upload(lists)
+    orc.finalize_initialization()
+    orc.keep_step_start(True)
+    energies = []
+    for t in range(steps):
+        orc.set_step_terms(every > 0 and t % every == 0)
+        orc.step(dt, 1)
+        energies.append(orc.interface_energy("previous") / lists.ncells)
+    return orc.download(), energies
+
+
+@pytest.mark.parametrize("every", [1, 2, 0])
+def test_driving_force_loop_matches_oracle(tmp_path, every):
+    """dG.Clear -> IP.Set -> DO(Phi, IP, dG) -> UDF.SetDrivingForce (steps divisible by every) ->
+    dG.Average -> dG.MergePhaseFieldIncrements -> Normalize -> Merge through the C++ classes: the
+    averaged driving force acts exactly in the steps whose driver set it."""
+    if not DG_EXAMPLE.exists():
+        pytest.fail("openphase_amd/examples/driving_force is not built")
+    steps = 12
+    cfg, lists, phase, dt = _dg_case(tmp_path, steps)
+    print(f"driving_force example every {every}", flush=True)
+    res = subprocess.run([str(DG_EXAMPLE), str(tmp_path / "in.opi"), str(tmp_path / "in.dat"),
+                          str(tmp_path / "grains.bin"), str(steps), str(tmp_path / "out.dat"), str(every)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    exp, energies = _oracle_run(cfg, lists, phase, dt, steps, every)
+    full = rawio.read_phasefield(str(tmp_path / "out.dat"))
+    keep = full.value != 0.0
+    cell = np.repeat(np.arange(full.ncells), full.count)
+    assert np.array_equal(np.bincount(cell[keep], minlength=full.ncells), exp.count)
+    assert np.array_equal(full.index[keep], exp.index)
+    # device tanh of the limiter: last-bit differences (as the averaged-force parity test)
+    assert np.abs(full.value[keep] - exp.value).max() <= 1e-12
+    got = _energies(res.stdout)
+    assert sorted(got) == list(range(steps))
+    for t in range(steps):
+        assert got[t] == pytest.approx(energies[t], rel=1e-12)
+    if every != 1:
+        # the skipped calls change the physics: not the run with the force in every step
+        ref_all, _ = _oracle_run(cfg, lists, phase, dt, steps, 1)
+        assert not (np.array_equal(ref_all.count, exp.count) and np.array_equal(ref_all.index, exp.index)
+                    and np.abs(ref_all.value - exp.value).max() < 1e-9)
+
+
+def test_normal_gg_ignores_an_unapplied_driving_force(tmp_path):
+    """NormalGG's loop (2-argument CalculatePhaseFieldIncrements, no DrivingForce calls) on an input
+    file that configures a UserDrivingForce: as in the reference, no driving force acts."""
+    if not EXAMPLE.exists():
+        pytest.fail("openphase_amd/examples/normal_gg is not built")
+    steps = 10
+    cfg, lists, phase, dt = _dg_case(tmp_path, steps)
+    res = subprocess.run([str(EXAMPLE), str(tmp_path / "in.opi"), str(tmp_path / "in.dat"), str(tmp_path / "grains.bin"),
+                          str(steps), str(tmp_path / "out.dat")], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    exp, _ = _oracle_run(cfg, lists, phase, dt, steps, 0)
+    full = rawio.read_phasefield(str(tmp_path / "out.dat"))
+    keep = full.value != 0.0
+    assert np.array_equal(full.index[keep], exp.index)
+    # grains below the reference volume: VolumeRatio's last bits follow the summation order
+    assert np.abs(full.value[keep] - exp.value).max() <= 1e-12
+    # the same loop with the force applied ends elsewhere
+    with_force, _ = _oracle_run(cfg, lists, phase, dt, steps, 1)
+    assert not (np.array_equal(with_force.index, exp.index) and np.abs(with_force.value - exp.value).max() < 1e-9)

@@ -4,8 +4,9 @@ import pytest
 
 from _golden import CASES, Case
 from openphase_amd import _capi
+from openphase_amd._capi import MpfError
 from openphase_amd.engine import Engine, StepConfig
-from openphase_amd.microstructure import voronoi_grains
+from openphase_amd.microstructure import random_rotations, voronoi_grains
 import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -66,8 +67,13 @@ def test_fp64_matches_golden(name, kernel):
                 assert eng.max_time_step() == case.max_dt[s - 1]
             else:
                 assert eng.max_time_step() == pytest.approx(case.max_dt[s - 1], rel=1e-12)
+            # DoubleObstacle::AverageEnergyDensity with the step's IP.Set energies; the device sums
+            # in another order than the reference's loop (rounding only)
+            assert eng.average_energy_density("previous") == pytest.approx(case.energy[s - 1], rel=1e-12)
     assert eng.limiter_warnings() == case.limiter_warnings
     assert eng.overflow_cell() == -1
+    assert eng.average_energy_density("current") == pytest.approx(case.energy_ip_current, rel=1e-12)
+    assert eng.interface_energy("none") == 0.0
     eng.close()
 
 
@@ -86,6 +92,7 @@ def test_fp64_matches_oracle(shape, ngrains, steps, kernel):
         e.set_grains(phase)
         e.upload(lists)
         e.finalize_initialization()
+    orc.keep_step_start(True)
     done = 0
     for s in (1, steps // 3, steps):
         eng.step(dt, s - done)
@@ -102,7 +109,68 @@ def test_fp64_matches_oracle(shape, ngrains, steps, kernel):
         np.testing.assert_allclose(gv, ov, rtol=1e-13, atol=1e-9)
         np.testing.assert_allclose(gr, orr, rtol=1e-13, atol=1e-15)
         assert eng.max_time_step() == orc.max_time_step()
+        for ip in ("previous", "current"):
+            assert eng.interface_energy(ip) == pytest.approx(orc.interface_energy(ip), rel=1e-12), f"step {s}: {ip}"
     assert eng.limiter_warnings() == orc.limiter_warnings()
+    eng.close()
+
+
+@pytest.mark.parametrize("case", ["cubic_dg", "averaged"])
+def test_step_terms_match_oracle(case):
+    """The driving force switched per step as a driver's calls decide (UserDrivingForce::
+    SetDrivingForce / DrivingForce::Average / MergePhaseFieldIncrements made or skipped): the GPU
+    step against the oracle, whose switch matches the reference's skipped calls bit for bit on the
+    golden case dg_every3_2d_32. Energies of the step's IP.Set (CUBIC: from the orientations)."""
+    if case == "cubic_dg":
+        q, rot = random_rotations(16, 31)
+        cfg = StepConfig(48, 40, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                         energy_model=np.ones((2, 2), np.int32), epsilon=np.full((2, 2), 0.05),
+                         dg_mode=np.array([[0, 1], [1, 0]], np.int32), dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]),
+                         slots=16)
+        phase = (np.arange(16) % 2).astype(np.int32)
+    else:
+        rot = None
+        cfg = _dg3(48, 48, 0, dg_average=True, slots=16)
+        phase = (np.arange(16) % 3).astype(np.int32)
+    lists = voronoi_grains(cfg.nx, cfg.ny, cfg.nz, 16, seed=41)
+    dt = 0.9 * cfg.max_time_step()
+    print(f"step terms {case}", flush=True)
+    eng = Engine(cfg)
+    orc = O.Oracle(cfg)
+    for e in (eng, orc):
+        e.set_grains(phase, rot)
+        e.upload(lists)
+        e.finalize_initialization()
+    orc.keep_step_start(True)
+    # (driving force, average) per step
+    pattern = [(1, 1), (0, 1), (0, 0), (1, 0), (1, 1), (0, 1), (1, 1), (1, 0), (0, 0), (1, 1)]
+    for s, (dg, avg) in enumerate(pattern, 1):
+        for e in (eng, orc):
+            e.set_step_terms(dg, avg)
+            e.step(dt, 1)
+        got, exp = eng.download(), orc.download()
+        _assert_same_sets(got, exp, f"step {s}")
+        err = np.abs(got.value - exp.value).max()
+        assert err <= FP64_TOL, f"step {s}: max |dphi| {err}"
+        assert eng.interface_energy("previous") == pytest.approx(orc.interface_energy("previous"), rel=1e-12)
+    assert eng.interface_energy("current") == pytest.approx(orc.interface_energy("current"), rel=1e-12)
+    eng.close()
+
+
+def test_step_terms_need_a_driving_force():
+    cfg = StepConfig(16, 16, 0)
+    eng = Engine(cfg)
+    eng.set_grains(np.zeros(4, np.int32))
+    eng.upload(voronoi_grains(16, 16, 0, 4, seed=3))
+    eng.finalize_initialization()
+    eng.set_step_terms(False, True)
+    with pytest.raises(MpfError):
+        eng.set_step_terms(True, True)
+    # the state the last IP.Set saw is held only once a step ran
+    with pytest.raises(MpfError):
+        eng.interface_energy("previous")
+    eng.step(cfg.max_time_step(), 1)
+    assert eng.interface_energy("previous") > 0.0
     eng.close()
 
 
