@@ -174,6 +174,12 @@ typedef enum mpf_ip_state {
  * a step since the last upload / restore / finalize. */
 mpf_status mpf_interface_energy(mpf_ctx* ctx, int32_t ip_state, double* sum);
 
+/* PhaseField::CalculateFractions (src/PhaseField.cpp:411-423), which the reference runs in every
+ * Finalize: per cell and thermodynamic phase the sum of its grains' values, computed on the
+ * device from the current state into dst[phase * cells + cell] (cells in x-outer, z-inner
+ * order): caller device memory of nphases * cells doubles, or host memory when dst_on_host. */
+mpf_status mpf_phase_fractions(mpf_ctx* ctx, double* dst, int32_t dst_on_host);
+
 /* Terms of the following steps (default: as created). driving_force 0 leaves the constant
  * driving force out: the reference applies it only in steps whose driver calls
  * UserDrivingForce::SetDrivingForce (src/UserDrivingForce.cpp:152-175), DrivingForce::Average

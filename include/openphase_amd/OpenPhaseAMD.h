@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdint>
 #include <fstream>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -281,6 +282,7 @@ class PhaseField {
     {
         ensure_context();
         check(mpf_upload(ctx_, count, index, value), ctx_, "PhaseField::Read");
+        fractions_step_ = -1;
     }
     // PhaseField::Write(FileName) (src/PhaseField.cpp:2340-2388): the lists as the reference holds
     // them after a step, zero-valued appended entries included
@@ -306,9 +308,10 @@ class PhaseField {
         io::Lists own;
         std::vector<int8_t> flag;
         Download(own.count, own.index, own.value, &flag);
+        const std::vector<double>& frac = Fractions.values();
         const std::string fn = io::make_file_name(s.VTKDir, "PhaseField_", tStep, ".vts");
         if (!io::write_vtk(fn, dims(), static_cast<int>(Nphases), grain_records(), io::full_lists(dims(), own, flag),
-                           flag, precision))
+                           flag, precision, &frac))
             throw std::runtime_error("PhaseField::WriteVTK: cannot write " + fn);
     }
     // GrainsProperties of the phase field: Write(FileName) (include/GrainsProperties.h:768-795)
@@ -318,6 +321,31 @@ class PhaseField {
         size_t size() const { return owner->grain_phase_.size(); }
     };
     GrainsTable FieldsProperties{this};
+    // PhaseField::Fractions (src/PhaseField.cpp:411-423, kept current by every Finalize): phase
+    // fractions per cell, computed on the device from the current state when first read after a
+    // step; Fractions(i, j, k, {n})
+    struct FractionsField {
+        const PhaseField* owner;
+        double operator()(int i, int j, int k, std::initializer_list<size_t> n) const
+        {
+            const GridParameters& G = owner->Grid;
+            const long ny = G.Ny ? G.Ny : 1, nz = G.Nz ? G.Nz : 1;
+            const long cell = (static_cast<long>(i) * ny + j) * nz + k;
+            return values()[*n.begin() * static_cast<size_t>(G.TotalNumberOfCells()) + static_cast<size_t>(cell)];
+        }
+        // [phase * cells + cell]
+        const std::vector<double>& values() const
+        {
+            PhaseField* p = const_cast<PhaseField*>(owner);
+            if (p->fractions_step_ != p->steps_ || p->fractions_.empty()) {
+                p->fractions_.assign(p->Nphases * static_cast<size_t>(p->Grid.TotalNumberOfCells()), 0.0);
+                check(mpf_phase_fractions(p->ctx_, p->fractions_.data(), 1), p->ctx_, "PhaseField::Fractions");
+                p->fractions_step_ = p->steps_;
+            }
+            return p->fractions_;
+        }
+    };
+    FractionsField Fractions{this};
     // Grain records as GrainsProperties holds them (device volumes, stages; host phase, orientation)
     std::vector<io::GrainRecord> grain_records() const
     {
@@ -361,6 +389,7 @@ class PhaseField {
         check_bc(BC);
         ensure_context();
         check(mpf_finalize_initialization(ctx_), ctx_, "PhaseField::FinalizeInitialization");
+        fractions_step_ = -1;
         stage_ = 0;
     }
     // PhaseField::NormalizeIncrements (src/PhaseField.cpp:1334-1349): third call of a step
@@ -439,6 +468,8 @@ class PhaseField {
     int dg_step_ = 0;    // driving force merged into this step: 0 none, 1 raw (SkipAverage), 2 averaged
     long steps_ = 0;     // steps run on the device
     long ip_step_ = -1;  // steps_ at the last IP.Set on this phase field (-1: none)
+    std::vector<double> fractions_;  // Fractions downloaded at step fractions_step_
+    long fractions_step_ = -1;
 };
 
 // ------------------------------------------------------------------------------------------

@@ -223,9 +223,11 @@ inline bool read_grains(const std::string& path, std::vector<GrainRecord>& grain
 
 // PhaseField::WriteVTK (single resolution, CurvatureOutput false): an ASCII structured grid with
 // Interfaces, Flags, PhaseFields (majority index), PhaseFraction_n, Junctions (list length with the
-// appended zero-valued entries), Variants, ParentGrain; `full` from full_lists.
+// appended zero-valued entries), Variants, ParentGrain; `full` from full_lists. `fractions`
+// ([phase * cells + cell], PhaseField::Fractions) when given, else summed here from `full`.
 inline std::string vtk_text(const Dims& d, int nphases, const std::vector<GrainRecord>& grains, const Lists& full,
-                            const std::vector<int8_t>& flag, int precision = 16)
+                            const std::vector<int8_t>& flag, int precision = 16,
+                            const std::vector<double>* fractions = nullptr)
 {
     const long N = d.cells();
     const std::vector<long> off = full.offsets();
@@ -277,6 +279,7 @@ inline std::string vtk_text(const Dims& d, int nphases, const std::vector<GrainR
         as_double();
         header("PhaseFraction_" + std::to_string(n), "Float64");
         each([&](long c) {
+            if (fractions) return (*fractions)[static_cast<size_t>(n) * N + c];
             double f = 0.0;
             for (long q = off[c]; q < off[c + 1]; ++q)
                 if (static_cast<int>(grains[full.index[q]].phase) == n) f += full.value[q];
@@ -314,16 +317,16 @@ inline std::string vtk_text(const Dims& d, int nphases, const std::vector<GrainR
     b << "</Points>\n";
     b << "</StructuredGrid>\n";
     b << "</VTKFile>\n";
-    (void)N;
     return b.str();
 }
 
 inline bool write_vtk(const std::string& path, const Dims& d, int nphases, const std::vector<GrainRecord>& grains,
-                      const Lists& full, const std::vector<int8_t>& flag, int precision = 16)
+                      const Lists& full, const std::vector<int8_t>& flag, int precision = 16,
+                      const std::vector<double>* fractions = nullptr)
 {
     std::ofstream out(path);
     if (!out) return false;
-    out << vtk_text(d, nphases, grains, full, flag, precision);
+    out << vtk_text(d, nphases, grains, full, flag, precision, fractions);
     return static_cast<bool>(out);
 }
 

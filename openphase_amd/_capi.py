@@ -30,7 +30,7 @@ EXPORTED = [
     "mpf_create", "mpf_set_grains", "mpf_upload", "mpf_finalize_initialization", "mpf_step",
     "mpf_step_async", "mpf_synchronize", "mpf_total_entries", "mpf_download", "mpf_get_grains",
     "mpf_get_grain_records",
-    "mpf_max_time_step", "mpf_interface_energy", "mpf_set_step_terms", "mpf_limiter_warnings",
+    "mpf_max_time_step", "mpf_interface_energy", "mpf_phase_fractions", "mpf_set_step_terms", "mpf_limiter_warnings",
     "mpf_overflow_cell", "mpf_device_state",
     "mpf_last_timing", "mpf_enable_timing", "mpf_step_stats", "mpf_copy_bandwidth", "mpf_set_pair_params",
     "mpf_state_bytes", "mpf_save_state", "mpf_restore_state", "mpf_adjoint_step", "mpf_autotune",
@@ -98,6 +98,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_max_time_step": ([vp, f64p], C.c_int),
         "mpf_interface_energy": ([vp, C.c_int32, f64p], C.c_int),
         "mpf_set_step_terms": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "mpf_phase_fractions": ([vp, f64p, C.c_int32], C.c_int),
         "mpf_limiter_warnings": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "mpf_overflow_cell": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "mpf_device_state": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)], C.c_int),

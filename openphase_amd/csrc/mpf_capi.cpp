@@ -119,6 +119,8 @@ struct mpf_ctx {
     bool prev_valid = false;
     double* d_energy = nullptr;           // per-block partial sums of the interface energy, then the sum
     size_t energy_cap = 0;
+    double* d_fractions = nullptr;        // phase fractions for host downloads
+    size_t fractions_cap = 0;
 
     int xcd_order = 1;               // launch shape, see StepArgs
     int deferred_grid = mpf::kWideGrid;
@@ -926,6 +928,29 @@ mpf_status mpf_interface_energy(mpf_ctx* c, int32_t ip_state, double* sum)
     return st;
 }
 
+mpf_status mpf_phase_fractions(mpf_ctx* c, double* dst, int32_t dst_on_host)
+{
+    if (!c || !dst) return MPF_ERR_ARG;
+    if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
+    const size_t n = static_cast<size_t>(c->nphases) * static_cast<size_t>(c->ncells);
+    double* out = dst;
+    if (dst_on_host) {
+        if (c->fractions_cap < n) {
+            dfree(c->d_fractions);
+            c->fractions_cap = 0;
+            if (c->hip(dalloc(&c->d_fractions, n), "phase fractions")) return MPF_ERR_NOMEM;
+            c->fractions_cap = n;
+        }
+        out = c->d_fractions;
+    }
+    const mpf::StepArgs a = step_args(c, 1.0, c->cur);
+    mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_fractions(a, c->fp32, out, c->stream)), "fractions kernel");
+    if (!st && dst_on_host)
+        st = c->hip(hipMemcpyAsync(dst, out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "fractions download");
+    if (!st) st = c->hip(hipStreamSynchronize(c->stream), "stream synchronize");
+    return st;
+}
+
 mpf_status mpf_set_step_terms(mpf_ctx* c, int32_t driving_force, int32_t average)
 {
     if (!c) return MPF_ERR_ARG;
@@ -1035,7 +1060,7 @@ void mpf_destroy(mpf_ctx* c)
     dfree(c->d_args); dfree(c->d_first); dfree(c->d_gargs); dfree(c->d_gfirst);
     dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed); dfree(c->d_dgmode_off);
-    dfree(c->d_energy);
+    dfree(c->d_energy); dfree(c->d_fractions);
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
     dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats); dfree(c->d_obs);
     dfree(c->d_avg_tmp); dfree(c->d_avg); dfree(c->d_avg_off); dfree(c->d_avg_off_w);

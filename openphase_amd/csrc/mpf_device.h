@@ -172,6 +172,8 @@ struct EnergyArgs {
 int64_t energy_blocks(int64_t ncells);
 // Writes the sum over all cells to `out` (device) after the per-block sums.
 int launch_energy(const EnergyArgs& a, bool fp32, double* out, void* stream);
+// Per-cell phase fractions of the state (cnt_in, id_in, val_in): out[phase * ncells + cell].
+int launch_fractions(const StepArgs& a, bool fp32, double* out, void* stream);
 int launch_grains(const GrainArgs* args, const uint32_t* first, int ndom, unsigned blocks, void* stream);
 int launch_copy(const void* src, void* dst, int64_t bytes, void* stream);
 

@@ -1,4 +1,6 @@
-// Interface energy of the double-obstacle functional summed over the grid:
+// Quantities derived from the state on demand: per-cell phase fractions
+// (PhaseField::CalculateFractions, src/PhaseField.cpp:411-423), and the interface energy of the
+// double-obstacle functional summed over the grid:
 // DoubleObstacle::PointEnergy (src/DoubleObstacle.cpp:903-923) per cell, added up by
 // DoubleObstacle::Energy / AverageEnergyDensity (:953-967, :989-1005).
 //
@@ -98,7 +100,35 @@ __global__ __launch_bounds__(kEnergyBlock) void energy_total_kernel(const double
     if (threadIdx.x == 0) *out = s;
 }
 
+// One thread per cell: for every thermodynamic phase the sum of its grains' values in list order
+// (the FP32 implicit entry as 1 - the others), out[phase * ncells + cell].
+template <typename TV>
+__global__ __launch_bounds__(kEnergyBlock) void fractions_kernel(const StepArgs A, double* __restrict__ out)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kEnergyBlock + threadIdx.x;
+    if (c >= A.ncells) return;
+    CellList<kMaxSlots, double, false> L;
+    load_own<kMaxSlots, TV, double, false>(A, c, A.cnt_in[c], L);
+    for (int n = 0; n < A.nphases; ++n) {
+        double f = 0.0;
+        for (int t = 0; t < L.n0; ++t)
+            if (A.gphase[L.eid[t]] == n) f += L.phi[t];
+        out[static_cast<int64_t>(n) * A.ncells + c] = f;
+    }
+}
+
 }  // namespace
+
+int launch_fractions(const StepArgs& a, bool fp32, double* out, void* stream)
+{
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nb = static_cast<unsigned>((a.ncells + kEnergyBlock - 1) / kEnergyBlock);
+    if (fp32)
+        hipLaunchKernelGGL(fractions_kernel<float>, dim3(nb), dim3(kEnergyBlock), 0, s, a, out);
+    else
+        hipLaunchKernelGGL(fractions_kernel<double>, dim3(nb), dim3(kEnergyBlock), 0, s, a, out);
+    return static_cast<int>(hipGetLastError());
+}
 
 int64_t energy_blocks(int64_t ncells) { return (ncells + kEnergyBlock - 1) / kEnergyBlock; }
 

@@ -226,6 +226,12 @@ class Engine:
         """DoubleObstacle::AverageEnergyDensity (src/DoubleObstacle.cpp:989-1005)."""
         return self.interface_energy(ip) / float(np.prod(self.cfg.shape))
 
+    def phase_fractions(self) -> np.ndarray:
+        """PhaseField::Fractions computed on the device: [nphases, cells]."""
+        out = np.zeros((self.cfg.nphases, int(np.prod(self.cfg.shape))))
+        self._check(self.lib.mpf_phase_fractions(self.h, ptr(out, C.c_double), 1), "mpf_phase_fractions")
+        return out
+
     def set_step_terms(self, driving_force: bool = True, average: bool = True) -> None:
         """Driving force in the following steps or not (the reference's driver calls
         UserDrivingForce::SetDrivingForce / DrivingForce::Average / MergePhaseFieldIncrements),

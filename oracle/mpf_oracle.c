@@ -1192,3 +1192,17 @@ int orc_interface_energy(const orc_ctx* c, int32_t ip_state, double* sum)
     *sum = energy_sum(c, ipn);
     return ORC_OK;
 }
+
+/* PhaseField::CalculateFractions (src/PhaseField.cpp:411-423): per cell and thermodynamic phase
+ * the sum of its grains' values in list order, out[phase * ncells + cell]. */
+int orc_phase_fractions(const orc_ctx* c, double* out)
+{
+    if (!c || !out) return ORC_ERR_ARG;
+    const long N = c->ncells;
+    for (long i = 0; i < (long)c->nphases * N; ++i) out[i] = 0.0;
+    for (long ci = 0; ci < N; ++ci) {
+        const orc_node* nd = &c->nodes[ci];
+        for (int q = 0; q < nd->n; ++q) out[(long)c->gphase[nd->e[q].idx] * N + ci] += nd->e[q].val;
+    }
+    return ORC_OK;
+}

@@ -110,6 +110,8 @@ enum orc_ip_state { ORC_IP_NONE = 0, ORC_IP_CURRENT = 1, ORC_IP_PREVIOUS = 2 };
  * order: DoubleObstacle::AverageEnergyDensity = sum / cells (:989-1005), Energy = sum * cell
  * volume (:953-967). */
 int orc_interface_energy(const orc_ctx* c, int32_t ip_state, double* sum);
+/* PhaseField::CalculateFractions (src/PhaseField.cpp:411-423): out[phase * cells + cell]. */
+int orc_phase_fractions(const orc_ctx* c, double* out);
 /* Longest list seen after the neighbour merge (incl. zero-valued entries) and after finalize. */
 void orc_list_stats(const orc_ctx* c, int32_t* max_merged, int32_t* max_finalized);
 

@@ -64,6 +64,7 @@ def load():
         lib.orc_set_step_terms.argtypes = [vp, C.c_int32, C.c_int32]
         lib.orc_keep_step_start.argtypes = [vp, C.c_int32]
         lib.orc_interface_energy.argtypes = [vp, C.c_int32, f64p]
+        lib.orc_phase_fractions.argtypes = [vp, f64p]
         _lib = lib
     return _lib
 
@@ -205,6 +206,13 @@ class Oracle:
         out = C.c_double(0.0)
         self._check(self.lib.orc_interface_energy(self.h, self.IP_STATES[ip], C.byref(out)), "interface_energy")
         return out.value
+
+    def phase_fractions(self):
+        """PhaseField::Fractions: [nphases, cells] per-phase sums of the values."""
+        n = int(np.prod(self.cfg.shape))
+        out = np.zeros((self.cfg.nphases, n))
+        self._check(self.lib.orc_phase_fractions(self.h, _p(out, C.c_double)), "phase_fractions")
+        return out
 
     def close(self):
         if self.h:

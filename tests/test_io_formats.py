@@ -39,6 +39,39 @@ class IOCase:
         return lists, flags, engine.grain_records(), vol
 
 
+def vtk_fractions(vtk: bytes, cfg):
+    """PhaseFraction_n arrays of a VTK file as the printed strings, [nphases, cells] in cell order
+    (x outer, z inner; the file lists points x fastest)."""
+    nx, ny, nz = storage_shape(cfg.nx, cfg.ny, cfg.nz)
+    lines = vtk.decode().splitlines()
+    out = []
+    for n in range(cfg.nphases):
+        start = next(i for i, ln in enumerate(lines) if f'Name = "PhaseFraction_{n}"' in ln) + 1
+        vals = np.array(lines[start:start + nx * ny * nz], dtype=object).reshape(nz, ny, nx)
+        out.append(vals.transpose(2, 1, 0).reshape(-1))
+    return np.array(out)
+
+
+def _printed(x):
+    """As the VTK writer prints a Float64 (std::defaultfloat, precision 16)."""
+    return np.array(["%.16g" % v for v in x.reshape(-1)], dtype=object).reshape(x.shape)
+
+
+@pytest.mark.parametrize("name", IO_CASES)
+def test_oracle_fractions_match_reference_vtk(name):
+    """PhaseField::CalculateFractions of the oracle against the PhaseFraction_n fields of the
+    reference's own VTK file (printed digits, exactly)."""
+    case = IOCase(name)
+    orc = O.Oracle(case.cfg)
+    case.run(orc)
+    frac = orc.phase_fractions()
+    orc.close()
+    assert np.array_equal(_printed(frac), vtk_fractions(case.expect["vtk"], case.cfg))
+    # two phases: a planted mistake (the grain index instead of its phase) changes the digits
+    if case.cfg.nphases > 1:
+        assert not np.array_equal(_printed(frac[::-1]), vtk_fractions(case.expect["vtk"], case.cfg))
+
+
 def _check(got, exp, what):
     for k in ("dat", "grains", "vtk"):
         assert got[k] == exp[k], f"{what}: {k} differs ({len(got[k])} vs {len(exp[k])} bytes)"
@@ -87,3 +120,27 @@ def test_gpu_state_writes_reference_bytes(name, tmp_path):
     # same structure (lists incl. appended zeros, flags, junctions, grain records) and values within
     # the FP64 bar (grain volumes are summed in another order; cubic energies round differently)
     assert_files_close(got, case.expect, 1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", IO_CASES)
+def test_gpu_fractions_match_oracle(name):
+    """Per-cell phase fractions computed on the device (PhaseField::CalculateFractions) against
+    the oracle's (bit for bit when the states are) and the reference VTK's printed digits."""
+    from openphase_amd.engine import Engine
+
+    case = IOCase(name)
+    print(f"device fractions: {name}", flush=True)
+    eng = Engine(case.cfg)
+    lists, _, _, _ = case.run(eng)
+    got = eng.phase_fractions()
+    eng.close()
+    orc = O.Oracle(case.cfg)
+    exp_lists, _, _, _ = case.run(orc)
+    exp = orc.phase_fractions()
+    orc.close()
+    assert got.shape == exp.shape
+    assert np.abs(got - exp).max() <= 1e-12
+    if np.array_equal(lists.value, exp_lists.value):
+        assert np.array_equal(got, exp)
+        assert np.array_equal(_printed(got), vtk_fractions(case.expect["vtk"], case.cfg))
