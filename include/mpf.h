@@ -233,16 +233,23 @@ mpf_status mpf_state_bytes(mpf_ctx* ctx, int64_t* bytes);
 mpf_status mpf_save_state(mpf_ctx* ctx, void* device_dst);
 mpf_status mpf_restore_state(mpf_ctx* ctx, const void* device_src);
 
-/* Adjoint of one FP64 step from the current state S_t with time step dt (no reference
- * counterpart: the reference has no derivatives). All pointers are DEVICE memory:
- *   lam_next  [slots * ncells]  cotangent of the values of S_{t+1}, slot-major like the state;
- *   lam_prev  [slots * ncells]  written: cotangent of the values of S_t (0 beyond each count);
- *   grad      [3 * nphases^2]   accumulated: cotangents of the sigma, mu and driving-force
- *                               tables (entries as read, before any symmetrization).
+/* Adjoint of one step from the current state S_t with time step dt (no reference counterpart:
+ * the reference has no derivatives). All pointers are DEVICE memory:
+ *   lam_next    [slots * ncells]  cotangent of the values of S_{t+1}, slot-major like the state;
+ *   lam_prev    [slots * ncells]  written: cotangent of the values of S_t (0 beyond each count);
+ *   grad        [3 * nphases^2]   accumulated: cotangents of the sigma, mu and driving-force
+ *                                 tables (entries as read, before any symmetrization), summed over
+ *                                 the cells in a fixed order (the same from run to run);
+ *   next_count, next_index        S_{t+1}'s count bytes and grain indices (the layout of
+ *                                 mpf_device_state / mpf_save_state), mapping lam_next to the
+ *                                 entries by grain; both NULL: S_{t+1} is the step's own result
+ *                                 from S_t (FP64 only).
  * Derivatives follow the branches the forward step takes (clamps, erasures, limiter factors);
- * VolumeRatios are held fixed. Requires FP64, at most 4 phases and merged lists of at most 8
- * entries (MPF_ERR_UNSUPPORTED otherwise). */
-mpf_status mpf_adjoint_step(mpf_ctx* ctx, double dt, const double* lam_next, double* lam_prev, double* grad);
+ * VolumeRatios are held fixed. FP32 states are differentiated at their values in FP64. At most 4
+ * phases, merged lists of at most 8 entries and no driving-force averaging (MPF_ERR_UNSUPPORTED
+ * otherwise). */
+mpf_status mpf_adjoint_step(mpf_ctx* ctx, double dt, const double* lam_next, double* lam_prev, double* grad,
+                            const void* next_count, const void* next_index);
 
 /* Launch-shape autotuner (no reference counterpart): times `steps` steps of each candidate launch
  * shape - XCD-contiguous block order of the step kernel on/off, 256 / 1024 / 4096 / 16384 blocks

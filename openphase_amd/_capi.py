@@ -110,7 +110,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "mpf_state_bytes": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "mpf_save_state": ([vp, vp], C.c_int),
         "mpf_restore_state": ([vp, vp], C.c_int),
-        "mpf_adjoint_step": ([vp, C.c_double, vp, vp, vp], C.c_int),
+        "mpf_adjoint_step": ([vp, C.c_double, vp, vp, vp, vp, vp], C.c_int),
         "mpf_autotune": ([vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "mpf_set_launch_shape": ([vp, C.c_int32, C.c_int32], C.c_int),
         "mpf_set_tile_shape": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),

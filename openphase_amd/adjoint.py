@@ -1,13 +1,14 @@
 """Differentiable multiphase-field rollouts: a torch.autograd.Function over the step and its adjoint.
 
-`rollout(engine, params, initial, nsteps, dt, segment)` advances the FP64 state saved in `initial`
-by `nsteps` steps with the phase-pair tables `params` = [sigma, mu, driving force] (float64,
-shape [3, nphases, nphases]) and returns the final state's values (float64, [slots, ncells],
-slot-major, on the GPU) with its counts and grain indices. Gradients of any loss of those values
-flow back to `params` through the hand-written adjoint of the step (mpf_adjoint_step): the
-forward pass keeps one checkpoint per `segment` steps; the backward pass recomputes the states of
-each segment from its checkpoint and runs the adjoint steps through it in reverse (segment
-checkpointing: memory ~ nsteps/segment + segment states).
+`rollout(engine, params, initial, nsteps, dt, segment)` advances the state saved in `initial`
+(FP64 or FP32 storage, as the engine's) by `nsteps` steps with the phase-pair tables `params` =
+[sigma, mu, driving force] (float64, shape [3, nphases, nphases]) and returns the final state's
+values (float64, [slots, ncells], slot-major, on the GPU) with its counts and grain indices.
+Gradients of any loss of those values flow back to `params` through the hand-written adjoint of
+the step (mpf_adjoint_step): the forward pass keeps one checkpoint per `segment` steps; the
+backward pass recomputes the states of each segment from its checkpoint and runs the adjoint
+steps through it in reverse (segment checkpointing: memory ~ nsteps/segment + segment states).
+FP32 rollouts are differentiated at their FP32 states with FP64 arithmetic.
 
 The reference has no derivatives; the forward map is the reference's step (src/DoubleObstacle.cpp,
 src/PhaseField.cpp). Gradients are those of the branches the forward steps take (clamps, erasures
@@ -42,13 +43,12 @@ class DeviceStates:
     """Saved states of one Engine as torch uint8 tensors on the GPU."""
 
     def __init__(self, engine: Engine):
-        if engine.cfg.precision != 0:
-            raise ValueError("differentiable rollouts run on FP64 states")
         self.engine = engine
+        self.fp32 = engine.cfg.precision != 0
         self.nbytes = engine.state_bytes()
         n = int(np.prod(engine.cfg.shape))
         self.ncells, self.slots = n, engine.cfg.slots
-        self.layout = state_layout(n, self.slots, 8, engine.ngrains)
+        self.layout = state_layout(n, self.slots, 4 if self.fp32 else 8, engine.ngrains)
         if self.layout["total"] != self.nbytes:
             raise RuntimeError("saved-state layout mismatch")
 
@@ -63,12 +63,23 @@ class DeviceStates:
         self.engine.restore_state(blob.data_ptr())
 
     def views(self, blob: torch.Tensor):
-        """(counts uint8 [N], grain indices int32 [slots, N], values float64 [slots, N])."""
+        """(counts uint8 [N], grain indices int32 [slots, N], values [slots, N]): views into an FP64
+        blob; for FP32 storage decoded copies - counts without the implicit-slot bits, values in
+        float64 with single-entry lists (nothing stored) at 1."""
         L, n, s = self.layout, self.ncells, self.slots
         cnt = blob[L["cnt"]:L["cnt"] + n]
         ids = blob[L["id"]:L["id"] + 2 * s * n].view(torch.int16).view(s, n).to(torch.int32) & 0xFFFF
-        val = blob[L["val"]:L["val"] + 8 * s * n].view(torch.float64).view(s, n)
+        if not self.fp32:
+            val = blob[L["val"]:L["val"] + 8 * s * n].view(torch.float64).view(s, n)
+            return cnt, ids, val
+        cnt = cnt & 15
+        val = blob[L["val"]:L["val"] + 4 * s * n].view(torch.float32).view(s, n).to(torch.float64)
+        val[0] = torch.where(cnt == 1, torch.ones_like(val[0]), val[0])
         return cnt, ids, val
+
+    def list_pointers(self, blob: torch.Tensor):
+        """Device addresses of a saved state's count bytes and grain indices (mpf_adjoint_step)."""
+        return blob.data_ptr() + self.layout["cnt"], blob.data_ptr() + self.layout["id"]
 
 
 class _Rollout(torch.autograd.Function):
@@ -88,7 +99,7 @@ class _Rollout(torch.autograd.Function):
         final = states.save()
         cnt, ids, val = states.views(final)
         val, cnt = val.clone(), cnt.clone()
-        ctx.states, ctx.checkpoints = states, checkpoints
+        ctx.states, ctx.checkpoints, ctx.final = states, checkpoints, final
         ctx.params = p.copy()
         ctx.nsteps, ctx.dt, ctx.segment = nsteps, dt, segment
         ctx.nphases = eng.cfg.nphases
@@ -116,6 +127,7 @@ class _Rollout(torch.autograd.Function):
         if g_val is not None:
             lam.copy_(g_val)
         nseg = len(ctx.checkpoints)
+        prev = torch.empty_like(lam)
         for si in reversed(range(nseg)):
             t0 = si * ctx.segment
             n = min(ctx.segment, ctx.nsteps - t0)
@@ -124,12 +136,13 @@ class _Rollout(torch.autograd.Function):
             for _ in range(n - 1):
                 eng.step(ctx.dt, 1)
                 seg_states.append(states.save())
+            seg_states.append(ctx.checkpoints[si + 1] if si + 1 < nseg else ctx.final)
             for i in reversed(range(n)):
                 states.restore(seg_states[i])
-                prev = torch.empty_like(lam)
+                nc, ni = states.list_pointers(seg_states[i + 1])
                 torch.cuda.synchronize()
-                eng.adjoint_step(ctx.dt, lam.data_ptr(), prev.data_ptr(), grad.data_ptr())
-                lam = prev
+                eng.adjoint_step(ctx.dt, lam.data_ptr(), prev.data_ptr(), grad.data_ptr(), nc, ni)
+                lam, prev = prev, lam
         ctx.lam0 = lam
         g = grad.view(3, ctx.nphases, ctx.nphases).cpu()
         return g, None, None, None, None, None

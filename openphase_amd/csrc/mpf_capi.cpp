@@ -105,6 +105,11 @@ struct mpf_ctx {
     uint16_t* a_id = nullptr;
     double* a_lap = nullptr;
     double* a_grad = nullptr;
+    double* a_partial = nullptr;      // per-block table cotangents
+    uint8_t* a_defer = nullptr;       // cells on the work list
+    uint32_t* a_list = nullptr;       // work list of cells with longer merged lists
+    unsigned* a_list_count = nullptr;
+    double* a_cellg = nullptr;        // per listed cell table cotangents
 
     int32_t* d_status = nullptr;
     unsigned long long* d_first_overflow = nullptr;
@@ -1056,7 +1061,8 @@ void mpf_destroy(mpf_ctx* c)
     for (auto e : c->events) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b) { dfree(c->cnt[b]); dfree(c->id[b]); dfree(c->val[b]); }
     free_grains(c);
-    dfree(c->a_lam_own); dfree(c->a_cnt); dfree(c->a_id); dfree(c->a_lap); dfree(c->a_grad);
+    dfree(c->a_lam_own); dfree(c->a_cnt); dfree(c->a_id); dfree(c->a_lap); dfree(c->a_grad); dfree(c->a_partial); dfree(c->a_defer);
+    dfree(c->a_list); dfree(c->a_list_count); dfree(c->a_cellg);
     dfree(c->d_args); dfree(c->d_first); dfree(c->d_gargs); dfree(c->d_gfirst);
     dfree(c->d_sigma); dfree(c->d_mu); dfree(c->d_eps); dfree(c->d_maxe);
     dfree(c->d_emodel); dfree(c->d_dgmode); dfree(c->d_dgval); dfree(c->d_dgallowed); dfree(c->d_dgmode_off);
@@ -1181,31 +1187,44 @@ mpf_status mpf_restore_state(mpf_ctx* c, const void* src)
     return c->hip(e, "state restore");
 }
 
-mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, double* lam_prev, double* grad)
+mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, double* lam_prev, double* grad,
+                            const void* next_count, const void* next_index)
 {
     if (!c || !lam_next || !lam_prev || !grad) return MPF_ERR_ARG;
     if (!c->initialized) return c->fail(MPF_ERR_STATE, "mpf_finalize_initialization first");
-    if (c->fp32) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint runs on FP64 states");
+    if (!(dt > 0.0)) return c->fail(MPF_ERR_ARG, "dt must be > 0");
+    if ((next_count == nullptr) != (next_index == nullptr)) return c->fail(MPF_ERR_ARG, "give both next_count and next_index");
+    if (c->fp32 && !next_count) return c->fail(MPF_ERR_ARG, "FP32 states need the next state's lists (next_count, next_index)");
     if (avg_active(c)) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint takes the driving force without averaging");
-    if (c->nphases > 4) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint handles at most 4 phases");
+    if (c->nphases > mpf::kAdjPhases) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint handles at most 4 phases");
     const size_t N = static_cast<size_t>(c->ncells);
     constexpr size_t KA = mpf::kAdjSlots;
     if (!c->a_cnt) {
+        const size_t G = 3 * static_cast<size_t>(c->nphases) * c->nphases;
         if (c->hip(dalloc(&c->a_lam_own, KA * N), "adjoint arrays") || c->hip(dalloc(&c->a_cnt, N), "adjoint arrays") ||
-            c->hip(dalloc(&c->a_id, KA * N), "adjoint arrays") || c->hip(dalloc(&c->a_lap, KA * N), "adjoint arrays"))
+            c->hip(dalloc(&c->a_id, KA * N), "adjoint arrays") || c->hip(dalloc(&c->a_lap, KA * N), "adjoint arrays") ||
+            c->hip(dalloc(&c->a_defer, N), "adjoint arrays") || c->hip(dalloc(&c->a_list, N), "adjoint arrays") ||
+            c->hip(dalloc(&c->a_list_count, 1), "adjoint arrays") || c->hip(dalloc(&c->a_cellg, N * G), "adjoint arrays") ||
+            c->hip(dalloc(&c->a_partial, static_cast<size_t>(mpf::adjoint_blocks(c->ncells)) * G), "adjoint arrays"))
             return MPF_ERR_NOMEM;
         if (c->grad && c->hip(dalloc(&c->a_grad, 3 * KA * N), "adjoint arrays")) return MPF_ERR_NOMEM;
-        if (c->hip(hipMemset(c->a_cnt, 0, N), "adjoint arrays")) return MPF_ERR_HIP;
     }
     mpf::AdjointArgs a{};
     a.s = step_args(c, dt, c->cur);
     a.lam_next = lam_next;
+    a.next_cnt = static_cast<const uint8_t*>(next_count);
+    a.next_id = static_cast<const uint16_t*>(next_index);
     a.lam_prev = lam_prev;
     a.lam_own = c->a_lam_own;
     a.adj_cnt = c->a_cnt;
+    a.defer = c->a_defer;
+    a.list = c->a_list;
+    a.list_count = c->a_list_count;
+    a.cellg = c->a_cellg;
     a.adj_id = c->a_id;
     a.adj_lap = c->a_lap;
     a.adj_grad = c->a_grad;
+    a.partial = c->a_partial;
     a.grad = grad;
     a.dg_limit = c->dg_limit;
     a.regularization = c->regularization;
@@ -1213,7 +1232,7 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
     a.driving_force = dg_active(c) ? 1 : 0;
     const int32_t zeros[mpf::kStatusWords] = {0, 0, 0, 0};
     mpf_status st = c->hip(hipMemcpyAsync(c->d_status, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream), "status reset");
-    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_adjoint(a, c->stream)), "adjoint kernels");
+    if (!st) st = c->hip(static_cast<hipError_t>(mpf::launch_adjoint(a, c->fp32, c->stream)), "adjoint kernels");
     int32_t words[mpf::kStatusWords];
     if (!st) st = c->hip(hipMemcpyAsync(words, c->d_status, sizeof(words), hipMemcpyDeviceToHost, c->stream), "status readback");
     if (!st) st = c->hip(hipStreamSynchronize(c->stream), "stream synchronize");

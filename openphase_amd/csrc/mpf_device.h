@@ -93,12 +93,19 @@ struct GrainArgs {
 struct AdjointArgs {
     StepArgs s;
     const double* lam_next;      // [slots][ncells] cotangent of the values of S_{t+1}
+    const uint8_t* next_cnt;     // count bytes of S_{t+1} (null: S_{t+1} holds the entries the
+    const uint16_t* next_id;     // step keeps, in list order); [slots][ncells] its grains
     double* lam_prev;            // [slots][ncells] cotangent of the values of S_t (written)
     double* lam_own;             // [kAdjSlots][ncells] direct part per own entry
     uint8_t* adj_cnt;            // [ncells] merged entries of an interface cell (0: bulk)
+    uint8_t* defer;              // [ncells] 1: the cell is on the work list
+    uint32_t* list;              // [ncells] work list: cells of three or more merged entries
+    unsigned* list_count;        // its length
+    double* cellg;               // [ncells][3 nphases^2] table cotangents of listed cells
     uint16_t* adj_id;            // [kAdjSlots][ncells]
     double* adj_lap;             // [kAdjSlots][ncells] cotangent of the Laplacians
     double* adj_grad;            // [3 kAdjSlots][ncells] cotangent of the gradients (may be null)
+    double* partial;             // [blocks][3 nphases^2] per-block table cotangents
     double* grad;                // [3][nphases^2] cotangents of sigma, mu, driving force (accumulated)
     double dg_limit;
     double regularization;
@@ -106,8 +113,10 @@ struct AdjointArgs {
     int32_t driving_force;       // constant driving force present
 };
 constexpr int kAdjSlots = 8;     // merged entries per cell the adjoint handles
+constexpr int kAdjPhases = 4;    // thermodynamic phases the adjoint handles
 
-int launch_adjoint(const AdjointArgs& a, void* stream);
+int64_t adjoint_blocks(int64_t ncells);
+int launch_adjoint(const AdjointArgs& a, bool fp32, void* stream);
 
 constexpr int kGrainBlock = 256;                     // threads per block of the grain kernel
 constexpr int kTileHalo = 396;                       // cells of a tile plane with its halo, at most (6 x 66)

@@ -63,8 +63,8 @@ template <> __device__ __forceinline__ double buffer_load<double>(__amdgpu_buffe
 }
 
 // std::min / std::max semantics (first argument wins ties and unordered compares).
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
 // Fixed-point image V = v * 2^116 of a non-negative v < 2^12 as four 32-bit limbs (exact for
 // v >= 2^-64; smaller parts are truncated).
@@ -160,7 +160,7 @@ __device__ __forceinline__ void note_maxima(unsigned long long* obs, double emax
 // InterfaceOrientation (src/InterfaceProperties.cpp:39-45) and the cubic energy of
 // InterfaceEnergyModel::Calculate (include/InterfaceProperties/InterfaceEnergyModel.h:325-345).
 template <typename TC>
-__device__ __forceinline__ TC cubic_energy(const double* R, TC gx, TC gy, TC gz, TC energy, TC eps1)
+__host__ __device__ __forceinline__ TC cubic_energy(const double* R, TC gx, TC gy, TC gz, TC energy, TC eps1)
 {
     TC v0 = gx * TC(-1.0), v1 = gy * TC(-1.0), v2 = gz * TC(-1.0);
     TC len = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
@@ -183,7 +183,7 @@ __device__ __forceinline__ TC cubic_energy(const double* R, TC gx, TC gy, TC gz,
 // the weights, MaxEnergy when neither grain weighs (CUBIC). ga/gb: grains, va/vb: values,
 // da/db: gradients of the two entries.
 template <typename TC>
-__device__ __forceinline__ TC ip_pair_energy(const StepArgs& A, int k, uint32_t ga, uint32_t gb, TC va, TC vb,
+__host__ __device__ __forceinline__ TC ip_pair_energy(const StepArgs& A, int k, uint32_t ga, uint32_t gb, TC va, TC vb,
                                              const TC (&da)[3], const TC (&db)[3])
 {
     if (A.emodel[k] == 0) return static_cast<TC>(A.sigma[k]);

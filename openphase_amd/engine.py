@@ -305,10 +305,13 @@ class Engine:
     def restore_state(self, device_ptr: int) -> None:
         self._check(self.lib.mpf_restore_state(self.h, C.c_void_p(device_ptr)), "mpf_restore_state")
 
-    def adjoint_step(self, dt: float, lam_next: int, lam_prev: int, grad: int) -> None:
-        """One adjoint step from the current state (device pointers, see mpf_adjoint_step)."""
+    def adjoint_step(self, dt: float, lam_next: int, lam_prev: int, grad: int, next_count: int | None = None,
+                     next_index: int | None = None) -> None:
+        """One adjoint step from the current state (device pointers, see mpf_adjoint_step); next_count /
+        next_index: S_{t+1}'s count bytes and grain indices (required for FP32 states)."""
         self._check(self.lib.mpf_adjoint_step(self.h, float(dt), C.c_void_p(lam_next), C.c_void_p(lam_prev),
-                                              C.c_void_p(grad)), "mpf_adjoint_step")
+                                              C.c_void_p(grad), C.c_void_p(next_count), C.c_void_p(next_index)),
+                    "mpf_adjoint_step")
 
     # ---------------------------------------------------------------- launch shape
     def autotune(self, dt: float, steps: int = 3):
