@@ -168,13 +168,14 @@ def run_fp32_accuracy(lists, checks=(100, 500, 1000)):
     return out
 
 
-def run_adjoint(steps: int, segment: int):
+def run_adjoint(steps: int, segment: int, precision: int):
     """Differentiable rollout of the calibration workload: 2D 512^2 periodic, 200 grains of two
     phases (random half/half), uniform random cubic orientations, cubic anisotropy 0.05,
     sigma 0.24, mu 4e-9, constant driving force 1e5 J/m^3 between the phases, dt = 0.9 x the 2D
-    limit, FP64. Forward `steps` steps with a checkpoint every `segment` steps, mismatch loss
-    against a target grown with tables perturbed by exp(0.1 N(0,1)), then the adjoint pass
-    (segment recomputation included). Returns (forward + adjoint) steps / wall second."""
+    limit, FP32 or FP64 states. Forward `steps` steps with a checkpoint every `segment` steps,
+    mismatch loss against a target grown with tables perturbed by exp(0.1 N(0,1)), then the
+    adjoint pass (segment recomputation included). Returns (forward + adjoint) steps / wall
+    second, the forward and backward rates apart, and the adjoint step's own device time."""
     import torch
     from openphase_amd import _capi
     from openphase_amd.adjoint import DeviceStates, dense_fields, mismatch_loss, rollout
@@ -185,7 +186,7 @@ def run_adjoint(steps: int, segment: int):
     cfg = StepConfig(n, n, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
                      energy_model=np.full((2, 2), _capi.MPF_ENERGY_CUBIC, np.int32), epsilon=np.full((2, 2), 0.05),
                      dg_mode=np.array([[0, 1], [1, 0]], np.int32), dg_value=np.array([[0.0, 1e5], [-1e5, 0.0]]),
-                     slots=8)
+                     slots=8, precision=precision)
     dt = 0.9 * cfg.max_time_step()
     rng = np.random.default_rng(0x0FE4)
     phase = np.zeros(ng, np.int32)
@@ -215,17 +216,45 @@ def run_adjoint(steps: int, segment: int):
     loss.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    # device time of one forward step and one adjoint step at the rollout's final state
+    fin = st.save()
+    st.restore(fin)
+    lam = torch.randn(st.slots, st.ncells, dtype=torch.float64, device="cuda")
+    prev = torch.empty_like(lam)
+    grad = torch.zeros(12, dtype=torch.float64, device="cuda")
+    eng.step(dt, 1)
+    nxt = st.save()
+    nc, ni = st.list_pointers(nxt)
+    st.restore(fin)
+    reps = 20
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    eng.step(dt, reps)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    st.restore(fin)
+    for _ in range(reps):
+        eng.adjoint_step(dt, lam.data_ptr(), prev.data_ptr(), grad.data_ptr(), nc, ni)
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
     eng.close()
+    fwd_ms, adj_ms = 1e3 * (t4 - t3) / reps, 1e3 * (t5 - t4) / reps
     return {
         "steps_per_s": 2 * steps / (t2 - t0),
         "forward_steps_per_s": steps / (t1 - t0),
-        "adjoint_steps_per_s": steps / (t2 - t1),
+        "backward_steps_per_s": steps / (t2 - t1),
+        "forward_step_ms": fwd_ms,
+        "adjoint_step_ms": adj_ms,
+        "adjoint_vs_forward_speed": fwd_ms / adj_ms,
         "steps": steps,
         "segment": segment,
+        "precision": "fp32" if precision else "fp64",
         "loss": float(loss.detach()),
-        "grad_norm": float(params.grad.norm()),
+        "grad": [float(x) for x in params.grad.reshape(-1)],
         "workload": "2D 512^2 periodic, 200 grains, 2 phases, cubic anisotropy 0.05, driving force 1e5 J/m^3, "
-                    "FP64 forward + adjoint with segment checkpointing (steps/s = (forward + adjoint steps) / wall)",
+                    "forward + adjoint with segment checkpointing (steps/s = (forward + adjoint steps) / wall; "
+                    "the backward pass recomputes each segment's states from its checkpoint); step times at the "
+                    "rollout's final state",
     }
 
 
@@ -375,7 +404,7 @@ def main():
     ap.add_argument("--no-fp32-accuracy", action="store_true", help="skip the FP32-vs-FP64 error leg")
     ap.add_argument("--only-main", action="store_true",
                     help="only the 256^3 FP64 / FP32 step legs (for profiling runs)")
-    ap.add_argument("--adjoint-steps", type=int, default=100, help="rollout length of the adjoint leg")
+    ap.add_argument("--adjoint-steps", type=int, default=500, help="rollout length of the adjoint leg")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the ensemble leg")
     ap.add_argument("--dg-steps", type=int, default=200, help="steps of the driving-force averaging leg")
     args = ap.parse_args()
@@ -397,8 +426,10 @@ def main():
     print(f"[bench] fp32 accuracy {acc32}", file=sys.stderr, flush=True)
     adj = ens = dga = cpu = None
     if not args.only_main:
-        adj = run_adjoint(args.adjoint_steps, 25)
-        print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
+        adj = {"fp32": run_adjoint(args.adjoint_steps, 50, 1)}
+        print(f"[bench] adjoint fp32 {adj['fp32']}", file=sys.stderr, flush=True)
+        adj["fp64"] = run_adjoint(args.adjoint_steps, 50, 0)
+        print(f"[bench] adjoint fp64 {adj['fp64']}", file=sys.stderr, flush=True)
         ens = run_ensemble(args.ensemble_steps)
         print(f"[bench] ensemble {ens}", file=sys.stderr, flush=True)
         dga = run_dg_average(args.dg_steps)

@@ -16,7 +16,9 @@
 //     increments psi (pair energies with the cubic anisotropy, triple-junction and driving-force
 //     terms). Longer lists go to a work list that adjoint_list_kernel takes in dense wavefronts
 //     (the list length C a compile-time constant), sweeping the sum-to-one limiter backwards
-//     through its iterations, recomputed from the increments entering it.
+//     through its iterations, recomputed from the increments entering it; lists of five or more
+//     entries (a few cells per step) go on to a second list and adjoint_long_kernel, so that
+//     the private-memory code of the long lists holds up only its own wavefronts.
 //     Out: the direct part per own entry, the cotangents of each merged entry's Laplacian and
 //     gradient, and the table cotangents - per block of cells, in thread order, and per listed
 //     cell, folded into its block's sum in cell order by adjoint_fold_kernel.
@@ -44,6 +46,7 @@ enum AdjError { kAdjCapacity = 3 };  // status word: a cell needed more than kAd
 
 constexpr int kAdjMid = 4;         // list capacity of the work-list kernel's register path
 constexpr int kAdjListBlocks = 256;  // blocks of the work-list kernel (grid-stride)
+constexpr int kAdjLongBlocks = 64;   // blocks of the second work-list kernel (grid-stride)
 
 // Writes the adjoint outputs of a wide cell: merged entries with the cotangents of their
 // Laplacians and gradients, and the direct part of the own entries.
@@ -103,32 +106,54 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_local_kernel(const AdjointA
     }
 }
 
-// One listed cell: lists of at most kAdjMid entries in registers, longer ones (rare) with
-// kAdjSlots. Cotangents stored; its table cotangents to J.cellg[cell].
+// One listed cell of at most kAdjMid merged entries (lists in registers): cotangents stored, its
+// table cotangents to J.cellg[cell]. A longer list goes to the second work list.
 template <typename TV, bool GRAD>
 __device__ __forceinline__ void list_cell(const AdjointArgs& J, int64_t c, double* gsl)
 {
     const StepArgs& A = J.s;
     const bool fp32 = sizeof(TV) == 4;
     const int G = 3 * A.nphases * A.nphases;
+    constexpr int K = kAdjMid;
+    CellList<K, double, GRAD> L;
+    const int r = gather_cell<K, TV, double, GRAD>(A, c, L);
+    if (r != kCellWide) {
+        J.list_long[atomicAdd(J.long_count, 1u)] = static_cast<uint32_t>(c);
+        return;
+    }
     for (int e = 0; e < G; ++e) gsl[e] = 0.0;
-    bool ok = false;
+    double lam_phi[K], lam_lap[K], lam_gr[GRAD ? K : 1][3];
+    switch (L.ne) {
+        case 2: cell_adjoint<2, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
+        case 3: cell_adjoint<3, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
+        default: cell_adjoint<4, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
+    }
+    store_adjoint<K, GRAD>(J, c, L, lam_phi, lam_lap, lam_gr);
+    for (int e = 0; e < G; ++e) J.cellg[c * G + e] = gsl[e];
+}
+
+// A cell of the second work list: five merged entries in registers (almost all of them), up to
+// kAdjSlots in private memory; more is reported in the status word.
+template <typename TV, bool GRAD>
+__device__ __forceinline__ void long_cell(const AdjointArgs& J, int64_t c, double* gsl)
+{
+    const StepArgs& A = J.s;
+    const bool fp32 = sizeof(TV) == 4;
+    const int G = 3 * A.nphases * A.nphases;
+    for (int e = 0; e < G; ++e) gsl[e] = 0.0;
+    bool done = false;
     {
-        constexpr int K = kAdjMid;
+        constexpr int K = 5;
         CellList<K, double, GRAD> L;
         const int r = gather_cell<K, TV, double, GRAD>(A, c, L);
-        if (r == kCellWide) {
+        if (r == kCellWide && L.ne == 5) {
             double lam_phi[K], lam_lap[K], lam_gr[GRAD ? K : 1][3];
-            switch (L.ne) {
-                case 2: cell_adjoint<2, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
-                case 3: cell_adjoint<3, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
-                default: cell_adjoint<4, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
-            }
+            cell_adjoint<5, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl);
             store_adjoint<K, GRAD>(J, c, L, lam_phi, lam_lap, lam_gr);
-            ok = true;
+            done = true;
         }
     }
-    if (!ok) {
+    if (!done) {
         constexpr int K = kAdjSlots;
         CellList<K, double, GRAD> L;
         const int r = gather_cell<K, TV, double, GRAD>(A, c, L);
@@ -138,7 +163,6 @@ __device__ __forceinline__ void list_cell(const AdjointArgs& J, int64_t c, doubl
         } else {
             double lam_phi[K], lam_lap[K], lam_gr[GRAD ? K : 1][3];
             switch (L.ne) {
-                case 5: cell_adjoint<5, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
                 case 6: cell_adjoint<6, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
                 case 7: cell_adjoint<7, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
                 default: cell_adjoint<8, GRAD, K>(J, fp32, c, L, lam_phi, lam_lap, lam_gr, gsl); break;
@@ -149,7 +173,7 @@ __device__ __forceinline__ void list_cell(const AdjointArgs& J, int64_t c, doubl
     for (int e = 0; e < G; ++e) J.cellg[c * G + e] = gsl[e];
 }
 
-// Work-list kernel: the cells of three or more merged entries in dense wavefronts.
+// Work-list kernel: the cells of three or four merged entries in dense wavefronts.
 template <typename TV, bool GRAD>
 __global__ __launch_bounds__(kAdjBlock) void adjoint_list_kernel(const AdjointArgs J)
 {
@@ -159,6 +183,18 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_list_kernel(const AdjointAr
     const unsigned n = *J.list_count;
     for (unsigned i = blockIdx.x * kAdjBlock + threadIdx.x; i < n; i += gridDim.x * kAdjBlock)
         list_cell<TV, GRAD>(J, J.list[i], gsl);
+}
+
+// Second work-list kernel: the few cells of five or more merged entries.
+template <typename TV, bool GRAD>
+__global__ __launch_bounds__(kAdjBlock) void adjoint_long_kernel(const AdjointArgs J)
+{
+    extern __shared__ double gsh[];
+    const int G = 3 * J.s.nphases * J.s.nphases;
+    double* gsl = gsh + static_cast<int>(threadIdx.x) * G;
+    const unsigned n = *J.long_count;
+    for (unsigned i = blockIdx.x * kAdjBlock + threadIdx.x; i < n; i += gridDim.x * kAdjBlock)
+        long_cell<TV, GRAD>(J, J.list_long[i], gsl);
 }
 
 // One wavefront per main-kernel block: adds the table cotangents of the block's listed cells to
@@ -218,7 +254,7 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_gather_kernel(const Adjoint
 
 __global__ void adjoint_reset_kernel(unsigned* count)
 {
-    if (threadIdx.x == 0) *count = 0u;
+    if (threadIdx.x < 2) count[threadIdx.x] = 0u;
 }
 
 // One block per table entry e: grad[e] += sum over blocks of partial[block][e], in block order per
@@ -248,6 +284,8 @@ void launch_all(const AdjointArgs& a, hipStream_t s)
     hipLaunchKernelGGL(adjoint_reset_kernel, dim3(1), dim3(64), 0, s, a.list_count);
     hipLaunchKernelGGL((adjoint_local_kernel<TV, GRAD>), grid, dim3(kAdjBlock), kAdjBlock * G * sizeof(double), s, a);
     hipLaunchKernelGGL((adjoint_list_kernel<TV, GRAD>), dim3(kAdjListBlocks), dim3(kAdjBlock),
+                       kAdjBlock * G * sizeof(double), s, a);
+    hipLaunchKernelGGL((adjoint_long_kernel<TV, GRAD>), dim3(kAdjLongBlocks), dim3(kAdjBlock),
                        kAdjBlock * G * sizeof(double), s, a);
     hipLaunchKernelGGL(adjoint_fold_kernel, grid, dim3(kAdjBlock), 0, s, a);
     hipLaunchKernelGGL((adjoint_gather_kernel<TV, GRAD>), grid, dim3(kAdjBlock), 0, s, a);

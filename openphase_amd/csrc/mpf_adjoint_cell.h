@@ -47,9 +47,9 @@ template <int C> struct Part2Rec {
 // increments by r2 = -(phi + pos dt) / (neg dt) (r1 = (1 - (phi + neg dt)) / (pos dt)); each pair
 // takes the smaller limit of its two entries. Returns whether a limit below 1 was applied.
 template <int C>
-MPF_HD inline bool limiter_pass(double (&psi)[NPairs<C>::value], const double (&phi)[C], double dt, uint32_t alive)
+MPF_HD __forceinline__ bool limiter_pass(double (&psi)[NPairs<C>::value], const double (&phi)[C], double dt, uint32_t alive)
 {
-    constexpr int UA = C <= 4 ? C : 1;
+    constexpr int UA = C <= 5 ? C : 1;
     bool needed = false;
     double pos[C], neg[C], lim1[C], lim2[C];
 #pragma unroll UA
@@ -95,11 +95,11 @@ MPF_HD inline bool limiter_pass(double (&psi)[NPairs<C>::value], const double (&
 // Limiter, merge and finalize of a cell of C entries (the FP64 part of the step kernels'
 // compute_cell). psi is modified; out gets the finalized values.
 template <int C>
-MPF_HD inline void part2(double dt, double (&psi)[NPairs<C>::value], const double (&phi)[C], double (&out)[C],
+MPF_HD __forceinline__ void part2(double dt, double (&psi)[NPairs<C>::value], const double (&phi)[C], double (&out)[C],
                          Part2Rec<C>& R)
 {
     constexpr int NPC = NPairs<C>::value;
-    constexpr int UA = C <= 4 ? C : 1;  // full unroll only for short lists (instruction cache)
+    constexpr int UA = C <= 5 ? C : 1;  // full unroll only for short lists (code size)
     uint32_t alive = 0u;
     R.limited = false;
     R.ne2 = 0;
@@ -239,11 +239,11 @@ MPF_HD inline double lam_of(const AdjointArgs& J, bool fp32, int64_t c, uint32_t
 // 0 (a symmetric configuration, e.g. equal energies of the pairs around it) by the mean of the
 // two one-sided derivatives, the limit of a central difference.
 template <int C>
-MPF_HD inline void limiter_reverse(const Part2Rec<C>& R, const double (&phi)[C], double dt,
+MPF_HD __forceinline__ void limiter_reverse(const Part2Rec<C>& R, const double (&phi)[C], double dt,
                                    double (&lam)[NPairs<C>::value], double (&lam_phi)[C])
 {
     constexpr int NPC = NPairs<C>::value;
-    constexpr int UA = C <= 4 ? C : 1;
+    constexpr int UA = C <= 5 ? C : 1;
 #pragma unroll UA
     for (int p = 0; p < NPC; ++p)
         if (!(R.live >> p & 1u)) lam[p] = 0.0;
@@ -358,10 +358,10 @@ template <int C, bool GRAD> struct CellFwd {
 };
 
 template <int C, bool GRAD, int K>
-MPF_HD void cell_forward(const AdjointArgs& J, const CellList<K, double, GRAD>& L, CellFwd<C, GRAD>& F)
+MPF_HD __forceinline__ void cell_forward(const AdjointArgs& J, const CellList<K, double, GRAD>& L, CellFwd<C, GRAD>& F)
 {
     constexpr int NPC = NPairs<C>::value;
-    constexpr int UA = C <= 4 ? C : 1;  // full unroll only for short lists (instruction cache)
+    constexpr int UA = C <= 5 ? C : 1;  // full unroll only for short lists (code size)
     const StepArgs& A = J.s;
     const int NP = A.nphases;
     const int n0 = L.n0;
@@ -441,11 +441,11 @@ MPF_HD void cell_forward(const AdjointArgs& J, const CellList<K, double, GRAD>& 
 
 // Reverse sweep of one wide cell whose merged list has C entries (list capacity K).
 template <int C, bool GRAD, int K>
-MPF_HD void cell_adjoint(const AdjointArgs& J, bool fp32, int64_t c, const CellList<K, double, GRAD>& L,
+MPF_HD __forceinline__ void cell_adjoint(const AdjointArgs& J, bool fp32, int64_t c, const CellList<K, double, GRAD>& L,
                          double (&lam_phi)[K], double (&lam_lap)[K], double (&lam_gr)[GRAD ? K : 1][3], double* gsl)
 {
     constexpr int NPC = NPairs<C>::value;
-    constexpr int UA = C <= 4 ? C : 1;  // full unroll only for short lists (instruction cache)
+    constexpr int UA = C <= 5 ? C : 1;  // full unroll only for short lists (code size)
     const StepArgs& A = J.s;
     const int NP = A.nphases, NP2 = NP * NP;
     const int n0 = L.n0;

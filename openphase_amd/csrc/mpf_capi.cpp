@@ -107,8 +107,8 @@ struct mpf_ctx {
     double* a_grad = nullptr;
     double* a_partial = nullptr;      // per-block table cotangents
     uint8_t* a_defer = nullptr;       // cells on the work list
-    uint32_t* a_list = nullptr;       // work list of cells with longer merged lists
-    unsigned* a_list_count = nullptr;
+    uint32_t* a_list = nullptr;       // work lists of cells with longer merged lists (two of N)
+    unsigned* a_list_count = nullptr; // their lengths
     double* a_cellg = nullptr;        // per listed cell table cotangents
 
     int32_t* d_status = nullptr;
@@ -1203,8 +1203,8 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
         const size_t G = 3 * static_cast<size_t>(c->nphases) * c->nphases;
         if (c->hip(dalloc(&c->a_lam_own, KA * N), "adjoint arrays") || c->hip(dalloc(&c->a_cnt, N), "adjoint arrays") ||
             c->hip(dalloc(&c->a_id, KA * N), "adjoint arrays") || c->hip(dalloc(&c->a_lap, KA * N), "adjoint arrays") ||
-            c->hip(dalloc(&c->a_defer, N), "adjoint arrays") || c->hip(dalloc(&c->a_list, N), "adjoint arrays") ||
-            c->hip(dalloc(&c->a_list_count, 1), "adjoint arrays") || c->hip(dalloc(&c->a_cellg, N * G), "adjoint arrays") ||
+            c->hip(dalloc(&c->a_defer, N), "adjoint arrays") || c->hip(dalloc(&c->a_list, 2 * N), "adjoint arrays") ||
+            c->hip(dalloc(&c->a_list_count, 2), "adjoint arrays") || c->hip(dalloc(&c->a_cellg, N * G), "adjoint arrays") ||
             c->hip(dalloc(&c->a_partial, static_cast<size_t>(mpf::adjoint_blocks(c->ncells)) * G), "adjoint arrays"))
             return MPF_ERR_NOMEM;
         if (c->grad && c->hip(dalloc(&c->a_grad, 3 * KA * N), "adjoint arrays")) return MPF_ERR_NOMEM;
@@ -1220,6 +1220,8 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
     a.defer = c->a_defer;
     a.list = c->a_list;
     a.list_count = c->a_list_count;
+    a.list_long = c->a_list + N;
+    a.long_count = c->a_list_count + 1;
     a.cellg = c->a_cellg;
     a.adj_id = c->a_id;
     a.adj_lap = c->a_lap;

@@ -101,6 +101,8 @@ struct AdjointArgs {
     uint8_t* defer;              // [ncells] 1: the cell is on the work list
     uint32_t* list;              // [ncells] work list: cells of three or more merged entries
     unsigned* list_count;        // its length
+    uint32_t* list_long;         // [ncells] second work list: cells of more than four merged entries
+    unsigned* long_count;        // its length
     double* cellg;               // [ncells][3 nphases^2] table cotangents of listed cells
     uint16_t* adj_id;            // [kAdjSlots][ncells]
     double* adj_lap;             // [kAdjSlots][ncells] cotangent of the Laplacians
