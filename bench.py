@@ -265,19 +265,15 @@ def run_ensemble(steps: int):
     ensemble (one captured graph per two steps of all members) and the same domains stepped one
     after another are timed. Returns total cell-updates/s of both."""
     from openphase_amd.engine import Engine, Ensemble, StepConfig
-    from openphase_amd.microstructure import voronoi_grains
+    from openphase_amd.microstructure import calibration_domains, voronoi_grains
 
-    rng = np.random.default_rng(0x0FE6)
     engines, dts, cells = [], [], 0
-    for i in range(64):
-        side = int(round(128 * 4 ** rng.uniform()))
-        ng = int(round(np.exp(rng.uniform(np.log(20), np.log(300)))))
-        sig = rng.uniform(0.1, 0.5)
-        mu = float(np.exp(rng.uniform(np.log(1e-9), np.log(1e-8))))
-        cfg = StepConfig(side, side, 0, sigma=np.array([[sig]]), mu=np.array([[mu]]), precision=1, slots=8)
+    for d in calibration_domains(64, 0x0FE6):
+        side, ng = d.side, d.ngrains
+        cfg = StepConfig(side, side, 0, sigma=np.array([[d.sigma]]), mu=np.array([[d.mu]]), precision=1, slots=8)
         e = Engine(cfg)
         e.set_grains(np.zeros(ng, np.int32))
-        e.upload(voronoi_grains(side, side, 0, ng, seed=0x0FE6 + i))
+        e.upload(voronoi_grains(side, side, 0, ng, seed=d.seed))
         e.finalize_initialization()
         dt = 0.9 * cfg.max_time_step()
         e.step(dt, 2)

@@ -251,11 +251,15 @@ mpf_status mpf_restore_state(mpf_ctx* ctx, const void* device_src);
 mpf_status mpf_adjoint_step(mpf_ctx* ctx, double dt, const double* lam_next, double* lam_prev, double* grad,
                             const void* next_count, const void* next_index);
 
-/* Launch-shape autotuner (no reference counterpart): times `steps` steps of each candidate launch
- * shape - XCD-contiguous block order of the step kernel on/off, 256 / 1024 / 4096 / 16384 blocks
- * for the deferred-cell kernels - on the context's own state (saved and restored around every trial, so
- * the state is unchanged afterwards) and keeps the fastest. Results do not depend on the shape.
- * The chosen shape is returned through the optional out pointers. */
+/* Launch-shape autotuner (no reference counterpart): times `steps` steps of each candidate on the
+ * context's own state (saved and restored around every trial, so the state is unchanged
+ * afterwards) and keeps the fastest, in two rounds: (1) the step kernel's tile (grids with two or
+ * three active axes) - plane tiles of 64-256 cells (2D: 64 / 128 / 256 along the active plane
+ * axis; 3D: 4x64, 8x32, 16x16, 4x32) times 1 / 4 / 16 / the default number of planes per block,
+ * and the per-cell kernel; (2) the XCD-contiguous block order of the step kernel on/off times
+ * 256 / 1024 / 4096 / 16384 blocks for the deferred-cell kernels. Results do not depend on the
+ * shape. The chosen launch shape is returned through the optional out pointers, the tile by
+ * mpf_get_tile_shape. */
 mpf_status mpf_autotune(mpf_ctx* ctx, double dt, int32_t steps, int32_t* xcd_order, int32_t* deferred_grid);
 mpf_status mpf_set_launch_shape(mpf_ctx* ctx, int32_t xcd_order, int32_t deferred_grid);
 

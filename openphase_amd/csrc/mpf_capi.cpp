@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -163,7 +164,7 @@ struct mpf_ctx {
 // Members of one kernel variant (precision, orientation/driving-force terms, equal-energy form)
 // share every launch: device arrays of their arguments for the two captured steps.
 struct mpf_group {
-    bool fp32 = false, full = false, uni = false;
+    bool fp32 = false, full = false, uni = false, tiled = false;
     std::vector<int> members;
     std::vector<mpf::StepArgs> h_a1, h_a2;
     std::vector<mpf::GrainArgs> h_g;
@@ -1270,26 +1271,76 @@ mpf_status mpf_autotune(mpf_ctx* c, double dt, int32_t steps, int32_t* xcd_order
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!st) st = c->hip(hipEventCreate(&e0), "hipEventCreate");
     if (!st) st = c->hip(hipEventCreate(&e1), "hipEventCreate");
-    const int grids[4] = {256, 1024, mpf::kWideGrid, 4 * mpf::kWideGrid};
+    // one trial: the saved state, a warm-up step, then `steps` timed steps
+    auto trial = [&](float* ms) -> mpf_status {
+        c->cur = cur0;
+        mpf_status t = mpf_restore_state(c, saved);
+        if (!t) t = enqueue_step(c, dt, c->stream);
+        if (!t) t = c->hip(hipEventRecord(e0, c->stream), "hipEventRecord");
+        for (int k = 0; k < steps && !t; ++k) t = enqueue_step(c, dt, c->stream);
+        if (!t) t = c->hip(hipEventRecord(e1, c->stream), "hipEventRecord");
+        if (!t) t = c->hip(hipEventSynchronize(e1), "hipEventSynchronize");
+        if (!t) t = c->hip(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime");
+        if (!t) t = read_status(c);
+        return t;
+    };
+    // 1. step-kernel tile: plane tiles of 64-256 cells x 1-16 planes per block (or the default
+    //    count), and the per-cell kernel
+    const int tx0 = c->tile_x, ty0 = c->tile_y, tz0 = c->tile_z;
     float best = 1e30f;
+    int best_tx = tx0, best_ty = ty0, best_tz = tz0;
+    if (c->active >= 2 && !st) {
+        std::vector<std::array<int, 2>> planes;
+        if (c->dny && c->dnz) planes = {{4, 64}, {8, 32}, {16, 16}, {4, 32}};
+        else if (c->dny) planes = {{64, 1}, {128, 1}, {256, 1}};
+        else planes = {{1, 64}, {1, 128}, {1, 256}};
+        std::vector<std::array<int, 3>> cands = {{0, 0, 0}};  // tile_x 0: the per-cell kernel
+        std::vector<std::array<int, 2>> seen;
+        for (auto pl : planes) {
+            // a tile wider than the grid is the grid's width
+            const int ty = std::min(pl[0], c->ny), tz = std::min(pl[1], c->nz);
+            if ((ty + 2 * c->dny) * (tz + 2 * c->dnz) > mpf::kTileHalo) continue;
+            if (std::find(seen.begin(), seen.end(), std::array<int, 2>{ty, tz}) != seen.end()) continue;
+            seen.push_back({ty, tz});
+            for (int xs : {0, 1, 4, 16}) {
+                int dy = 0, dz = 0, tx = 0;
+                tile_shape(c, xs, &dy, &dz, &tx);
+                cands.push_back({tx, ty, tz});
+            }
+        }
+        for (const auto& cd : cands) {
+            if (st) break;
+            c->tile_x = cd[0];
+            c->tile_y = cd[1];
+            c->tile_z = cd[2];
+            float ms = 0.f;
+            st = trial(&ms);
+            if (!st && ms < best) { best = ms; best_tx = cd[0]; best_ty = cd[1]; best_tz = cd[2]; }
+        }
+        if (st) { best_tx = tx0; best_ty = ty0; best_tz = tz0; }
+        c->tile_x = best_tx;
+        c->tile_y = best_ty;
+        c->tile_z = best_tz;
+    }
+    // 2. block order of the step kernel x grid of the deferred-cell kernels
+    const int grids[4] = {256, 1024, mpf::kWideGrid, 4 * mpf::kWideGrid};
+    best = 1e30f;
     int best_x = x0, best_g = g0;
     for (int x = 0; x < 2 && !st; ++x)
         for (int gi = 0; gi < 4 && !st; ++gi) {
             c->xcd_order = x;
             c->deferred_grid = grids[gi];
-            c->cur = cur0;
-            st = mpf_restore_state(c, saved);
-            if (!st) st = enqueue_step(c, dt, c->stream);  // warm-up
-            if (!st) st = c->hip(hipEventRecord(e0, c->stream), "hipEventRecord");
-            for (int k = 0; k < steps && !st; ++k) st = enqueue_step(c, dt, c->stream);
-            if (!st) st = c->hip(hipEventRecord(e1, c->stream), "hipEventRecord");
-            if (!st) st = c->hip(hipEventSynchronize(e1), "hipEventSynchronize");
             float ms = 0.f;
-            if (!st) st = c->hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-            if (!st) st = read_status(c);
+            st = trial(&ms);
             if (!st && ms < best) { best = ms; best_x = x; best_g = grids[gi]; }
         }
-    if (st) { best_x = x0; best_g = g0; }  // a failed trial keeps the previous shape
+    if (st) {  // a failed trial keeps the previous shape
+        best_x = x0;
+        best_g = g0;
+        c->tile_x = tx0;
+        c->tile_y = ty0;
+        c->tile_z = tz0;
+    }
     c->xcd_order = best_x;
     c->deferred_grid = best_g;
     c->cur = cur0;
@@ -1388,7 +1439,7 @@ bool build_args(mpf_ensemble* e, const double* dt)
             a2[k] = step_args(c, d, 1 - c->cur);
             a1[k].xcd_order = a2[k].xcd_order = 1;
             ga[k] = grain_args(c);
-            first[k + 1] = first[k] + step_blocks(c);
+            first[k + 1] = first[k] + (g.tiled ? tile_blocks(c) : step_blocks(c));
             gfirst[k + 1] = gfirst[k] + grain_blocks(c);
             // deferred-cell blocks per member: at most 256 (a 64-domain 2D ensemble: 7.0e9
             // cell-updates/s at 256, 6.8e9 at 512, 6.5e9 at 1024, 6.2e9 at 128; most blocks of a
@@ -1409,6 +1460,46 @@ bool build_args(mpf_ensemble* e, const double* dt)
         }
     }
     return changed;
+}
+
+// Group of a member: kernel variant (precision; a member with a driving force runs the full
+// variant whether mpf_set_step_terms switched it on or off - off: an all-zero mode table - so the
+// grouping holds across switches; uniform tables) and step kernel (tiled or per-cell).
+std::array<bool, 4> group_key(const mpf_ctx* c)
+{
+    return {c->fp32, c->grad || c->dg, c->uniform, c->tile_x > 0};
+}
+
+// Sorts the members into groups by group_key; drops the graph.
+void regroup(mpf_ensemble* e)
+{
+    drop_graph(e);
+    for (auto& g : e->groups) free_group(g);
+    e->groups.clear();
+    for (size_t i = 0; i < e->members.size(); ++i) {
+        const std::array<bool, 4> k = group_key(e->members[i]);
+        mpf_group* g = nullptr;
+        for (auto& q : e->groups)
+            if (std::array<bool, 4>{q.fp32, q.full, q.uni, q.tiled} == k) g = &q;
+        if (!g) {
+            e->groups.emplace_back();
+            g = &e->groups.back();
+            g->fp32 = k[0];
+            g->full = k[1];
+            g->uni = k[2];
+            g->tiled = k[3];
+        }
+        g->members.push_back(static_cast<int>(i));
+    }
+}
+
+// Whether a member's group key changed since the grouping (new tables, tile shape).
+bool regroup_needed(const mpf_ensemble* e)
+{
+    for (const auto& g : e->groups)
+        for (int m : g.members)
+            if (group_key(e->members[m]) != std::array<bool, 4>{g.fp32, g.full, g.uni, g.tiled}) return true;
+    return false;
 }
 
 // Uploads the groups' argument arrays and captures two steps of every group into one graph:
@@ -1445,8 +1536,10 @@ mpf_status capture(mpf_ensemble* e)
                 const mpf_ctx* c = e->members[g.members[k]];
                 if ((half == 0 ? g.h_a1[k] : g.h_a2[k]).dg_avg) rc = mpf::launch_dg_average(avg_args(c, half == 0 ? g.h_a1[k] : g.h_a2[k]), g.fp32, e->stream);
             }
-            if (!rc) rc = mpf::launch_step(half == 0 ? g.h_a1[0] : g.h_a2[0], args, g.d_first, n, blocks, g.fp32, g.full, g.uni,
-                                  e->stream);
+            if (!rc)
+                rc = g.tiled ? mpf::launch_tile_batch(args, g.d_first, n, blocks, g.fp32, g.full, g.uni, e->stream)
+                             : mpf::launch_step(half == 0 ? g.h_a1[0] : g.h_a2[0], args, g.d_first, n, blocks, g.fp32,
+                                                g.full, g.uni, e->stream);
             if (!rc) rc = mpf::launch_deferred(args, n, g.per_domain, g.fp32, g.full, e->stream);
             if (!rc) rc = mpf::launch_grains(g.d_g, g.d_gfirst, n, gblocks, e->stream);
             if (rc) break;
@@ -1481,23 +1574,7 @@ mpf_status mpf_ensemble_create(mpf_ctx* const* members, int32_t n, mpf_ensemble*
     mpf_ensemble* e = new (std::nothrow) mpf_ensemble();
     if (!e) return MPF_ERR_NOMEM;
     e->members.assign(members, members + n);
-    for (int32_t i = 0; i < n; ++i) {
-        const mpf_ctx* c = members[i];
-        // a member with a driving force runs the full variant whether mpf_set_step_terms switched
-        // it on or off (off: an all-zero mode table), so the grouping holds across switches
-        const bool full = c->grad || c->dg;
-        mpf_group* g = nullptr;
-        for (auto& q : e->groups)
-            if (q.fp32 == c->fp32 && q.full == full && q.uni == c->uniform) g = &q;
-        if (!g) {
-            e->groups.emplace_back();
-            g = &e->groups.back();
-            g->fp32 = c->fp32;
-            g->full = full;
-            g->uni = c->uniform;
-        }
-        g->members.push_back(i);
-    }
+    regroup(e);
     bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
     e->join.assign(n, nullptr);
     for (int32_t i = 0; i < n && ok; ++i) ok = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
@@ -1520,6 +1597,7 @@ mpf_status mpf_ensemble_step(mpf_ensemble* e, const double* dt, int32_t nsteps)
         if (hipEventRecord(e->join[i], e->members[i]->stream) != hipSuccess || hipStreamWaitEvent(e->stream, e->join[i], 0) != hipSuccess)
             return MPF_ERR_HIP;
     mpf_status st = MPF_OK;
+    if (regroup_needed(e)) regroup(e);
     if (nsteps >= 2 && (build_args(e, dt) || !e->exec)) st = capture(e);
     for (int32_t r = 0; r < nsteps / 2 && !st; ++r)
         if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) { e->err = "hipGraphLaunch failed"; st = MPF_ERR_HIP; }

@@ -125,6 +125,10 @@ constexpr int kTileHalo = 396;                       // cells of a tile plane wi
 
 // LDS-tiled step kernel over one domain (mpf_tile.hip); `blocks` = tiles x x-chunks.
 int launch_tile(const StepArgs& a, unsigned blocks, bool fp32, bool full, bool uni, void* stream);
+// The same over several domains (device array `args`, first[d] = the first block of domain d,
+// ndom + 1 prefix sums of the domains' tile counts), each with its own tile shape.
+int launch_tile_batch(const StepArgs* args, const uint32_t* first, int ndom, unsigned blocks, bool fp32, bool full,
+                      bool uni, void* stream);
 
 // Driving-force averaging (mpf_dgavg.hip): CollectAverage into `tmp`, DistributeAverage into
 // `avg` (= s.dg_avg), both [pairs][ncells] with pair a < b of the own entries at b(b-1)/2 + a.

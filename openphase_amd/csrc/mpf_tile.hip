@@ -135,8 +135,10 @@ __device__ __forceinline__ void tile_derivatives(const StepArgs& A, const uint32
     }
 }
 
+// One block of the tiled step: tile `b` of the domain A (tiles numbered z fastest, then y, then
+// the x-chunk).
 template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
-__global__ __launch_bounds__(kBlock, sizeof(TV) == 4 && !GRAD ? 4 : 1) void tile_kernel(const StepArgs A)
+__device__ __forceinline__ void tile_block(const StepArgs& A, const unsigned b)
 {
     __shared__ uint32_t s_info[3][kTileHalo];
     __shared__ uint32_t s_mask[3][kTileHalo];
@@ -159,7 +161,6 @@ __global__ __launch_bounds__(kBlock, sizeof(TV) == 4 && !GRAD ? 4 : 1) void tile
     const int TY = A.tile_y, TZ = A.tile_z;
     const int HZ = TZ + 2 * A.dnz, HP = (TY + 2 * A.dny) * HZ;
     const int nty = (NY + TY - 1) / TY, ntz = (NZ + TZ - 1) / TZ;
-    const unsigned b = A.xcd_order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int tz = static_cast<int>(b % static_cast<unsigned>(ntz));
     const int ty = static_cast<int>((b / static_cast<unsigned>(ntz)) % static_cast<unsigned>(nty));
     const int xc = static_cast<int>(b / static_cast<unsigned>(ntz * nty));
@@ -511,6 +512,22 @@ __global__ __launch_bounds__(kBlock, sizeof(TV) == 4 && !GRAD ? 4 : 1) void tile
     if (bad_volume) atomicOr(&A.status[kStatusVolumeRange], 1);
 }
 
+template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
+__global__ __launch_bounds__(kBlock, sizeof(TV) == 4 && !GRAD ? 4 : 1) void tile_kernel(const StepArgs A)
+{
+    tile_block<TV, TC, GRAD, DG, UNI>(A, A.xcd_order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
+// Several domains in one launch (ensembles): block b belongs to the domain d with
+// first[d] <= b < first[d+1], each with its own tile shape.
+template <typename TV, typename TC, bool GRAD, bool DG, bool UNI>
+__global__ __launch_bounds__(kBlock, sizeof(TV) == 4 && !GRAD ? 4 : 1) void tile_batch_kernel(
+    const StepArgs* __restrict__ batch, const uint32_t* __restrict__ first, int32_t ndom)
+{
+    const int dom = batch_domain(first, ndom, blockIdx.x);
+    tile_block<TV, TC, GRAD, DG, UNI>(batch[dom], blockIdx.x - first[dom]);
+}
+
 }  // namespace
 
 int launch_tile(const StepArgs& a, unsigned blocks, bool fp32, bool full, bool uni, void* stream)
@@ -518,6 +535,26 @@ int launch_tile(const StepArgs& a, unsigned blocks, bool fp32, bool full, bool u
     const dim3 grid(blocks);
     hipStream_t s = static_cast<hipStream_t>(stream);
 #define MPF_TILE(TV, G, D, U) hipLaunchKernelGGL((tile_kernel<TV, TV, G, D, U>), grid, dim3(kBlock), 0, s, a)
+    if (fp32) {
+        if (full)     MPF_TILE(float, true, true, false);
+        else if (uni) MPF_TILE(float, false, false, true);
+        else          MPF_TILE(float, false, false, false);
+    } else {
+        if (full)     MPF_TILE(double, true, true, false);
+        else if (uni) MPF_TILE(double, false, false, true);
+        else          MPF_TILE(double, false, false, false);
+    }
+#undef MPF_TILE
+    return static_cast<int>(hipGetLastError());
+}
+
+int launch_tile_batch(const StepArgs* args, const uint32_t* first, int ndom, unsigned blocks, bool fp32, bool full,
+                      bool uni, void* stream)
+{
+    const dim3 grid(blocks);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+#define MPF_TILE(TV, G, D, U) \
+    hipLaunchKernelGGL((tile_batch_kernel<TV, TV, G, D, U>), grid, dim3(kBlock), 0, s, args, first, ndom)
     if (fp32) {
         if (full)     MPF_TILE(float, true, true, false);
         else if (uni) MPF_TILE(float, false, false, true);

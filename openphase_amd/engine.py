@@ -315,8 +315,9 @@ class Engine:
 
     # ---------------------------------------------------------------- launch shape
     def autotune(self, dt: float, steps: int = 3):
-        """Times the candidate launch shapes on this state (restored afterwards) and keeps the
-        fastest; returns (xcd_order, deferred_grid)."""
+        """Times the candidate step-kernel tiles, then the launch shapes, on this state (restored
+        afterwards) and keeps the fastest (mpf_autotune); returns (xcd_order, deferred_grid), the
+        tile by tile_shape()."""
         x, g = C.c_int32(), C.c_int32()
         self._check(self.lib.mpf_autotune(self.h, float(dt), int(steps), C.byref(x), C.byref(g)), "mpf_autotune")
         return x.value, g.value

@@ -157,3 +157,30 @@ def quaternion_to_matrix(q: np.ndarray) -> np.ndarray:
     m[:, 7] = 2.0 * (y * z + s * x)
     m[:, 8] = 1.0 - 2.0 * (x * x + y * y)
     return m
+
+
+@dataclass(frozen=True)
+class CalibrationDomain:
+    """One member of a calibration ensemble: a periodic 2D domain side x side with `ngrains`
+    Voronoi grains (voronoi_grains(side, side, 0, ngrains, seed)) of one phase, isotropic
+    interface energy `sigma` (J/m^2) and mobility `mu` (m^4/(J s))."""
+    side: int
+    ngrains: int
+    sigma: float
+    mu: float
+    seed: int
+
+
+def calibration_domains(n: int = 64, seed: int = 0x0FE6) -> list:
+    """n domains with log-uniform sizes 128^2-512^2 (side round(128 * 4^U), U ~ U[0, 1]),
+    log-uniform 20-300 grains, sigma ~ U[0.1, 0.5] and mu ~ logU[1e-9, 1e-8]; member i takes the
+    Voronoi seed seed + i."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(round(128 * 4 ** rng.uniform()))
+        ng = int(round(np.exp(rng.uniform(np.log(20), np.log(300)))))
+        sig = float(rng.uniform(0.1, 0.5))
+        mu = float(np.exp(rng.uniform(np.log(1e-9), np.log(1e-8))))
+        out.append(CalibrationDomain(side, ng, sig, mu, seed + i))
+    return out
