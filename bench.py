@@ -294,6 +294,8 @@ def run_ensemble(steps: int):
             e.step(dt, 1)
     _device_sync()
     t_seq = time.perf_counter() - t0
+    adj = run_ensemble_adjoint(ens, engines, dts, steps, cells)
+    tiles = sorted({str(e.tile_shape()) for e in engines})
     ens.close()
     for e in engines:
         e.close()
@@ -303,8 +305,50 @@ def run_ensemble(steps: int):
         "domains": 64,
         "cells": cells,
         "steps": steps,
-        "workload": "64 periodic 2D domains, sides 128-512, 20-300 grains, per-domain sigma/mu/dt, FP32; "
-                    "one captured graph per two steps of all members vs the domains stepped one by one",
+        "tiles": tiles,
+        "adjoint": adj,
+        "workload": "64 periodic 2D domains, sides 128-512, 20-300 grains, per-domain sigma/mu/dt, FP32, every "
+                    "member autotuned (step-kernel tile, launch shape); one captured graph per two steps of all "
+                    "members vs the domains stepped one by one",
+    }
+
+
+def run_ensemble_adjoint(ens, engines, dts, steps: int, cells: int, segment: int = 50):
+    """Differentiable rollout of the whole ensemble (ensemble_rollout): `steps` steps of all
+    members together with a checkpoint of every member each `segment` steps, the sum of the
+    members' mismatch losses against their starting fields, and the reverse sweep of every member
+    (recomputation included). Returns the wall times and rates."""
+    import torch
+    from openphase_amd.adjoint import DeviceStates, dense_fields, ensemble_rollout, mismatch_loss
+
+    states = [DeviceStates(e) for e in engines]
+    inits = [st.save() for st in states]
+    targets = []
+    for st, init, e in zip(states, inits, engines):
+        c, i, v = st.views(init)
+        targets.append(dense_fields(v.clone(), c.clone(), i.clone(), e.ngrains))
+    p0 = np.stack([np.stack([e.pair_params["sigma"].reshape(1, 1), e.pair_params["mu"].reshape(1, 1), np.zeros((1, 1))])
+                   for e in engines])
+    params = torch.tensor(p0, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = ensemble_rollout(ens, states, params, inits, steps, dts, segment=segment)
+    loss = sum(mismatch_loss(v, c, i, t) for (v, c, i), t in zip(outs, targets))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {
+        "forward_s": t1 - t0,
+        "backward_s": t2 - t1,
+        "cell_updates_per_s": 2 * cells * steps / (t2 - t0),
+        "backward_cell_steps_per_s": cells * steps / (t2 - t1),
+        "segment": segment,
+        "loss": float(loss.detach()),
+        "grad_abs_max": float(params.grad.abs().max()),
+        "workload": "forward with checkpoints (all members together) + reverse sweep per member (segment "
+                    "recomputation included); cell_updates_per_s = (forward + adjoint cell-steps) / wall",
     }
 
 
@@ -401,7 +445,7 @@ def main():
     ap.add_argument("--only-main", action="store_true",
                     help="only the 256^3 FP64 / FP32 step legs (for profiling runs)")
     ap.add_argument("--adjoint-steps", type=int, default=500, help="rollout length of the adjoint leg")
-    ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the ensemble leg")
+    ap.add_argument("--ensemble-steps", type=int, default=200, help="steps of the ensemble leg")
     ap.add_argument("--dg-steps", type=int, default=200, help="steps of the driving-force averaging leg")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -16,6 +16,8 @@ and limiter factors as evaluated, VolumeRatios fixed).
 """
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import torch
 
@@ -54,12 +56,14 @@ class DeviceStates:
 
     def save(self) -> torch.Tensor:
         blob = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
-        torch.cuda.synchronize()
+        # torch's stream (earlier users of the memory) before the engine's stream writes it; the
+        # engine's copy has finished when save_state returns
+        torch.cuda.current_stream().synchronize()
         self.engine.save_state(blob.data_ptr())
         return blob
 
     def restore(self, blob: torch.Tensor) -> None:
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
         self.engine.restore_state(blob.data_ptr())
 
     def views(self, blob: torch.Tensor):
@@ -108,44 +112,110 @@ class _Rollout(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_val, g_cnt, g_ids):
-        states, eng = ctx.states, ctx.states.engine
-        # the recomputation and the adjoint steps run with this rollout's tables, whatever the
-        # engine holds now (another rollout may have run since); the caller's tables come back after
-        held = {k: (None if v is None else v.copy()) for k, v in eng.pair_params.items()}
-        p = ctx.params
-        eng.set_pair_params(p[0], p[1], p[2] if eng.cfg.dg_mode is not None else None)
-        try:
-            return _Rollout._backward(ctx, states, eng, g_val)
-        finally:
-            eng.set_pair_params(held["sigma"], held["mu"], held["dg_value"])
+        g, ctx.lam0 = _reverse(ctx.states, ctx.params, ctx.checkpoints, ctx.final, ctx.nsteps, ctx.dt, ctx.segment,
+                               g_val)
+        return g, None, None, None, None, None
 
-    @staticmethod
-    def _backward(ctx, states, eng, g_val):
-        npp = ctx.nphases * ctx.nphases
-        grad = torch.zeros(3 * npp, dtype=torch.float64, device="cuda")
+
+def _reverse(states: DeviceStates, p: np.ndarray, checkpoints, final, nsteps: int, dt: float, segment: int, g_val):
+    """Reverse sweep of one rollout of the engine behind `states` with tables p, from the
+    cotangent g_val of its final values: each segment's states are recomputed from its checkpoint,
+    then the adjoint steps run through it backwards. Returns (table cotangents [3, P, P] on the
+    CPU, cotangent of the initial values). The recomputation and the adjoint steps run with the
+    rollout's tables, whatever the engine holds now (another rollout may have run since); the
+    engine's tables come back afterwards."""
+    eng = states.engine
+    held = {k: (None if v is None else v.copy()) for k, v in eng.pair_params.items()}
+    eng.set_pair_params(p[0], p[1], p[2] if eng.cfg.dg_mode is not None else None)
+    try:
+        nph = eng.cfg.nphases
+        grad = torch.zeros(3 * nph * nph, dtype=torch.float64, device="cuda")
         lam = torch.zeros(states.slots, states.ncells, dtype=torch.float64, device="cuda")
         if g_val is not None:
             lam.copy_(g_val)
-        nseg = len(ctx.checkpoints)
+        # the engine's stream reads lam, prev and grad from here on; every engine call below returns
+        # after its own stream finished
+        torch.cuda.current_stream().synchronize()
+        nseg = len(checkpoints)
         prev = torch.empty_like(lam)
         for si in reversed(range(nseg)):
-            t0 = si * ctx.segment
-            n = min(ctx.segment, ctx.nsteps - t0)
-            states.restore(ctx.checkpoints[si])
-            seg_states = [ctx.checkpoints[si]]
+            t0 = si * segment
+            n = min(segment, nsteps - t0)
+            states.restore(checkpoints[si])
+            seg_states = [checkpoints[si]]
             for _ in range(n - 1):
-                eng.step(ctx.dt, 1)
+                eng.step(dt, 1)
                 seg_states.append(states.save())
-            seg_states.append(ctx.checkpoints[si + 1] if si + 1 < nseg else ctx.final)
+            seg_states.append(checkpoints[si + 1] if si + 1 < nseg else final)
             for i in reversed(range(n)):
                 states.restore(seg_states[i])
                 nc, ni = states.list_pointers(seg_states[i + 1])
-                torch.cuda.synchronize()
-                eng.adjoint_step(ctx.dt, lam.data_ptr(), prev.data_ptr(), grad.data_ptr(), nc, ni)
+                eng.adjoint_step(dt, lam.data_ptr(), prev.data_ptr(), grad.data_ptr(), nc, ni)
                 lam, prev = prev, lam
-        ctx.lam0 = lam
-        g = grad.view(3, ctx.nphases, ctx.nphases).cpu()
-        return g, None, None, None, None, None
+        return grad.view(3, nph, nph).cpu(), lam
+    finally:
+        eng.set_pair_params(held["sigma"], held["mu"], held["dg_value"])
+
+
+_REVERSE_THREADS = 8  # host threads driving the members' reverse sweeps of an ensemble rollout
+
+
+class _EnsembleRollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, ensemble, states, initials, nsteps, dts, segment):
+        p = params.detach().to("cpu", torch.float64).numpy()
+        for m, st in enumerate(states):
+            eng = st.engine
+            eng.set_pair_params(p[m][0], p[m][1], p[m][2] if eng.cfg.dg_mode is not None else None)
+            st.restore(initials[m])
+        checkpoints = [[] for _ in states]
+        done = 0
+        while done < nsteps:
+            for m, st in enumerate(states):
+                checkpoints[m].append(st.save())
+            n = min(segment, nsteps - done)
+            ensemble.step(list(dts), n)
+            done += n
+        finals = [st.save() for st in states]
+        outs = []
+        for st, fin in zip(states, finals):
+            cnt, ids, val = st.views(fin)
+            outs += [val.clone(), cnt.clone(), ids.clone()]
+        ctx.states, ctx.checkpoints, ctx.finals = states, checkpoints, finals
+        ctx.params = p.copy()
+        ctx.nsteps, ctx.dts, ctx.segment = nsteps, dts, segment
+        ctx.mark_non_differentiable(*outs[1::3], *outs[2::3])
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        # the members' reverse sweeps are independent and run on their engines' own streams: a
+        # few host threads keep several in flight on the GPU (the library calls release the GIL)
+        g = torch.zeros(ctx.params.shape, dtype=torch.float64)
+
+        def one(m):
+            gm, _ = _reverse(ctx.states[m], ctx.params[m], ctx.checkpoints[m], ctx.finals[m], ctx.nsteps, ctx.dts[m],
+                             ctx.segment, grads[3 * m])
+            return m, gm
+
+        torch.cuda.current_stream().synchronize()
+        with ThreadPoolExecutor(max_workers=min(_REVERSE_THREADS, len(ctx.states))) as pool:
+            for m, gm in pool.map(one, range(len(ctx.states))):
+                g[m] = gm
+        return g, None, None, None, None, None, None
+
+
+def ensemble_rollout(ensemble, states, params: torch.Tensor, initials, nsteps: int, dts, segment: int = 50):
+    """Differentiable rollout of every member of an Ensemble (engine.Ensemble): the forward pass
+    advances all members together (one captured graph per two steps of all members) and keeps a
+    checkpoint of every member each `segment` steps; the backward pass runs each member's reverse
+    sweep as `rollout` does. states: one DeviceStates per member, in the ensemble's order;
+    params: float64 [members, 3, P, P] (the members share the phase count P); initials: a saved
+    state per member; dts: a time step per member. Returns [(values, counts, grain ids)] per
+    member."""
+    outs = _EnsembleRollout.apply(params, ensemble, tuple(states), tuple(initials), int(nsteps),
+                                  tuple(float(d) for d in dts), int(segment))
+    return [tuple(outs[3 * m:3 * m + 3]) for m in range(len(states))]
 
 
 def rollout(engine_states: DeviceStates, params: torch.Tensor, initial: torch.Tensor, nsteps: int,

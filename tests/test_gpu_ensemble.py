@@ -179,3 +179,42 @@ def test_calibration_ensemble_matches_oracle(precision):
     ens.close()
     for e in members:
         e.close()
+
+
+def test_ensemble_rollout_gradients_match_single_rollouts():
+    """Gradients of a sum of per-member mismatch losses through ensemble_rollout (the members
+    advanced together, checkpoints every 5 steps, a reverse sweep per member) equal each member's
+    own rollout gradient bit for bit, FP32 and FP64 members alike."""
+    import torch
+
+    from openphase_amd.adjoint import DeviceStates, dense_fields, ensemble_rollout, mismatch_loss, rollout
+
+    doms = [d for d in _domains()]
+    steps, seg = 20, 5
+    team = [_make(c, l, g) for c, l, g, _ in doms]
+    dts = [d for *_, d in doms]
+    states = [DeviceStates(e) for e in team]
+    inits = [st.save() for st in states]
+    p0 = np.stack([np.stack([np.asarray(c.sigma, np.float64), np.asarray(c.mu, np.float64), np.zeros((1, 1))])
+                   for c, *_ in doms])
+    targets = []
+    with torch.no_grad():
+        for st, init, dt, (c, l, g, _) in zip(states, inits, dts, doms):
+            v, cn, i = rollout(st, torch.tensor(p0[len(targets)] * 1.1), init, steps, dt, segment=seg)
+            targets.append(dense_fields(v, cn, i, g))
+    ens = Ensemble(team)
+    params = torch.tensor(p0, requires_grad=True)
+    outs = ensemble_rollout(ens, states, params, inits, steps, dts, segment=seg)
+    loss = sum(mismatch_loss(v, cn, i, t) for (v, cn, i), t in zip(outs, targets))
+    loss.backward()
+    g_ens = params.grad.numpy().copy()
+    for m, (st, init, dt, t) in enumerate(zip(states, inits, dts, targets)):
+        pm = torch.tensor(p0[m], requires_grad=True)
+        v, cn, i = rollout(st, pm, init, steps, dt, segment=seg)
+        mismatch_loss(v, cn, i, t).backward()
+        print(f"member {m}: ensemble {g_ens[m].reshape(-1)[:2]}, alone {pm.grad.numpy().reshape(-1)[:2]}", flush=True)
+        assert np.array_equal(g_ens[m], pm.grad.numpy()), m
+        assert np.abs(g_ens[m]).max() > 0
+    ens.close()
+    for e in team:
+        e.close()
