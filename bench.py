@@ -72,7 +72,18 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
     eng.synchronize()
     _device_sync()
     wall = time.perf_counter() - t0
-    c_out = eng.download().count
+    t1 = time.perf_counter()
+    st_out = eng.download()
+    t_down = time.perf_counter() - t1
+    c_out = st_out.count
+    up = Engine(cfg)
+    up.set_grains(np.zeros(NGRAINS, np.int32))
+    _device_sync()
+    t1 = time.perf_counter()
+    up.upload(st_out)
+    _device_sync()
+    t_up = time.perf_counter() - t1
+    up.close()
     # second pass with per-kernel HIP events on the library's stream (same number of steps)
     eng.enable_timing(True)
     eng.step(dt, steps)
@@ -96,6 +107,11 @@ def run_gpu(precision: int, lists, steps: int, warmup: int):
         "achieved_gbs": b_alg / (k_step_ms * 1e-3) / 1e9 if k_step_ms > 0 else None,
         "copy_gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9,
         "deferred_cells_per_step": ((d_mid - d_wide) / total_steps, d_wide / total_steps),
+        # host buffers cross PCIe only in upload / download (the state as lists, host-side packing
+        # included); a caller that hands over host lists every step would see this rate
+        "upload_ms": 1e3 * t_up,
+        "download_ms": 1e3 * t_down,
+        "pcie_inclusive_cell_updates_per_s": ncell / (t_up + wall / steps + t_down),
     }
 
 
@@ -444,6 +460,8 @@ def main():
     ap.add_argument("--no-fp32-accuracy", action="store_true", help="skip the FP32-vs-FP64 error leg")
     ap.add_argument("--only-main", action="store_true",
                     help="only the 256^3 FP64 / FP32 step legs (for profiling runs)")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="only the adjoint leg, FP64 then FP32 (for profiling runs); prints its JSON")
     ap.add_argument("--adjoint-steps", type=int, default=500, help="rollout length of the adjoint leg")
     ap.add_argument("--ensemble-steps", type=int, default=200, help="steps of the ensemble leg")
     ap.add_argument("--dg-steps", type=int, default=200, help="steps of the driving-force averaging leg")
@@ -455,6 +473,11 @@ def main():
 
     from openphase_amd.microstructure import voronoi_grains
 
+    if args.only_adjoint:
+        seg = min(50, args.adjoint_steps)
+        print(json.dumps({"fp64": run_adjoint(args.adjoint_steps, seg, 0),
+                          "fp32": run_adjoint(args.adjoint_steps, seg, 1)}))
+        return
     t0 = time.perf_counter()
     lists = voronoi_grains(NX, NY, NZ, NGRAINS, seed=SEED, sides="plus")
     print(f"[bench] microstructure {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -530,6 +553,13 @@ def main():
             "fp32_step_kernel_ms": r32["kernel_ms"],
             "fp32_bytes_per_step": r32["bytes_per_step"],
             "copy_kernel_gbs": r64["copy_gbs"],
+        },
+        "host_transfer": {
+            "fp64_upload_ms": r64["upload_ms"],
+            "fp64_download_ms": r64["download_ms"],
+            "fp64_pcie_inclusive_cell_updates_per_s": r64["pcie_inclusive_cell_updates_per_s"],
+            "note": "state handed over as host lists before and after every step (mpf_upload / mpf_download, "
+                    "host-side packing included); value is device-resident",
         },
         "adjoint": adj,
         "ensemble": ens,
