@@ -9,7 +9,7 @@
 // src/PhaseField.cpp:1351-1567), the merge (:1268-1299) and NodePF::finalize
 // (include/Containers/NodePF.h:214-255). Bulk cells keep their single entry.
 //
-// Given the cotangent of the values of S_{t+1}, three kernels return the cotangent of the values
+// Given the cotangent of the values of S_{t+1}, three stages return the cotangent of the values
 // of S_t and add the cotangents of the pair tables:
 //  1. adjoint_local_kernel, one thread per cell: the cell update of a two-entry list (in
 //     registers) is recomputed in FP64 and swept backwards by hand - finalize and merge, then the
@@ -17,8 +17,9 @@
 //     terms). Longer lists go to a work list that adjoint_list_kernel takes in dense wavefronts
 //     (the list length C a compile-time constant), sweeping the sum-to-one limiter backwards
 //     through its iterations, recomputed from the increments entering it; lists of five or more
-//     entries (a few cells per step) go on to a second list and adjoint_long_kernel, so that
-//     the private-memory code of the long lists holds up only its own wavefronts.
+//     entries (a few cells per step) go on to a second list and adjoint_long_kernel (five
+//     entries in registers, six to eight in private memory), so that their long code holds up
+//     only its own wavefronts.
 //     Out: the direct part per own entry, the cotangents of each merged entry's Laplacian and
 //     gradient, and the table cotangents - per block of cells, in thread order, and per listed
 //     cell, folded into its block's sum in cell order by adjoint_fold_kernel.
