@@ -213,6 +213,14 @@ def ensemble_rollout(ensemble, states, params: torch.Tensor, initials, nsteps: i
     params: float64 [members, 3, P, P] (the members share the phase count P); initials: a saved
     state per member; dts: a time step per member. Returns [(values, counts, grain ids)] per
     member."""
+    engines = [st.engine for st in states]
+    if len(engines) != len(ensemble.engines) or any(x is not y for x, y in zip(engines, ensemble.engines)):
+        raise ValueError("states must hold the ensemble's members, in the ensemble's order")
+    nph = {e.cfg.nphases for e in engines}
+    if len(nph) != 1 or tuple(params.shape) != (len(states), 3, nph.pop(), engines[0].cfg.nphases):
+        raise ValueError("params must be [members, 3, P, P] with the phase count P the members share")
+    if len(initials) != len(states) or len(dts) != len(states):
+        raise ValueError("one initial state and one time step per member")
     outs = _EnsembleRollout.apply(params, ensemble, tuple(states), tuple(initials), int(nsteps),
                                   tuple(float(d) for d in dts), int(segment))
     return [tuple(outs[3 * m:3 * m + 3]) for m in range(len(states))]

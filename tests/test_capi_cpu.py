@@ -52,3 +52,30 @@ def test_invalid_arguments_fail_without_a_gpu():
     assert lib.mpf_create(None, None, C.byref(h)) == 1
     assert lib.mpf_step(None, 1e-3, 1) == 1
     assert lib.mpf_error_message(None) == b"null context"
+
+
+def test_ensemble_rollout_rejects_mismatched_arguments():
+    """Argument checks of ensemble_rollout run before any device work."""
+    import types
+
+    import pytest
+    import torch
+
+    from openphase_amd.adjoint import ensemble_rollout
+
+    def member(nph):
+        eng = types.SimpleNamespace(cfg=types.SimpleNamespace(nphases=nph))
+        return types.SimpleNamespace(engine=eng)
+
+    a, b = member(1), member(1)
+    ens = types.SimpleNamespace(engines=[a.engine, b.engine])
+    p = torch.zeros(2, 3, 1, 1, dtype=torch.float64)
+    with pytest.raises(ValueError, match="ensemble's order"):
+        ensemble_rollout(ens, [b, a], p, [None, None], 2, [1.0, 1.0])
+    with pytest.raises(ValueError, match=r"\[members, 3, P, P\]"):
+        ensemble_rollout(ens, [a, b], torch.zeros(2, 3, 2, 2, dtype=torch.float64), [None, None], 2, [1.0, 1.0])
+    with pytest.raises(ValueError, match="one initial state"):
+        ensemble_rollout(ens, [a, b], p, [None], 2, [1.0, 1.0])
+    c = member(2)
+    with pytest.raises(ValueError, match="phase count"):
+        ensemble_rollout(types.SimpleNamespace(engines=[a.engine, c.engine]), [a, c], p, [None, None], 2, [1.0, 1.0])
