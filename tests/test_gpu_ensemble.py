@@ -218,3 +218,39 @@ def test_ensemble_rollout_gradients_match_single_rollouts():
     ens.close()
     for e in team:
         e.close()
+
+
+@pytest.mark.parametrize("shape", [(40, 48, 56), (24, 300, 1)])
+def test_every_autotuner_tile_gives_the_same_result(shape):
+    """Each step-kernel tile the autotuner may choose (3D: 4x64, 8x32, 16x16, 4x32 plane tiles; 2D:
+    64 / 128 / 256 cells, capped at the grid; 1 / 4 / 16 planes per block) and the per-cell kernel
+    give bit-identical states, on grids whose sides are no multiples of the tiles."""
+    nx, ny, nz = shape
+    cfg = StepConfig(nx, ny, nz if nz > 1 else 0, slots=16)
+    ng = 40 if nz > 1 else 12
+    lists = voronoi_grains(nx, ny, nz if nz > 1 else 0, ng, seed=21)
+    dt = cfg.max_time_step()
+    ref = _make(cfg, lists, ng)
+    ref.set_tile_shape(-1)
+    ref.step(dt, 6)
+    if nz > 1:
+        planes = [(4, 64), (8, 32), (16, 16), (4, 32)]
+    else:
+        planes = [(64, 1), (128, 1), (256, 1)]
+    tried = 0
+    for ty, tz in planes:
+        ty, tz = min(ty, ny), min(tz, max(nz, 1))
+        for tx in (1, 4, 16, 0):
+            e = _make(cfg, lists, ng)
+            e.set_tile_shape(tx, ty, tz)
+            e.step(dt, 6)
+            _same(ref, e)
+            e.close()
+            tried += 1
+    tuned = _make(cfg, lists, ng)
+    tuned.autotune(dt, steps=1)
+    tuned.step(dt, 6)
+    _same(ref, tuned)
+    print(f"{shape}: {tried} tiles and the tuned one ({tuned.tile_shape()}) match the per-cell kernel", flush=True)
+    tuned.close()
+    ref.close()
