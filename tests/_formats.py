@@ -14,17 +14,19 @@ GRAIN_DTYPE = np.dtype([("exist", "u1"), ("stage", "<i4"), ("phase", "<i4"), ("s
                         ("refvolume", "<f8"), ("volume", "<f8"), ("maxvolume", "<f8"), ("q", "<f8", (4,))])
 
 
-def build_driver(tmp: Path) -> Path:
-    exe = tmp / "io_driver"
+def build_driver(tmp: Path, flags=(), name: str = "io_driver") -> Path:
+    exe = tmp / name
     if not exe.exists():
-        subprocess.run(["g++", "-std=c++17", "-O2", "-I", str(ROOT / "include"), str(ROOT / "tests" / "io_driver.cpp"),
-                        "-o", str(exe)], check=True)
+        subprocess.run(["g++", "-std=c++17", "-O2", *flags, "-I", str(ROOT / "include"),
+                        str(ROOT / "tests" / "io_driver.cpp"), "-o", str(exe)], check=True)
     return exe
 
 
-def write_files(tmp: Path, cfg, lists, flags, phase, quaternions, records, volume) -> dict:
-    """records = (MAXVolume, RefVolume, Exist, Stage). Returns {"dat", "grains", "vtk"} bytes."""
-    exe = build_driver(tmp)
+def write_files(tmp: Path, cfg, lists, flags, phase, quaternions, records, volume, exe: Path | None = None,
+                env=None) -> dict:
+    """records = (MAXVolume, RefVolume, Exist, Stage). Returns {"dat", "grains", "vtk"} bytes.
+    exe: a driver build of build_driver (default: the plain one)."""
+    exe = exe or build_driver(tmp)
     rawio.write_phasefield(str(tmp / "state.dat"), lists)
     np.asarray(flags, np.int8).tofile(tmp / "flags.bin")
     mv, rv, ex, st = records
@@ -37,7 +39,7 @@ def write_files(tmp: Path, cfg, lists, flags, phase, quaternions, records, volum
         f.write(np.uint32(n).tobytes())
         f.write(g.tobytes())
     subprocess.run([str(exe), str(cfg.nx), str(cfg.ny), str(cfg.nz), str(cfg.nphases), str(tmp / "state.dat"),
-                    str(tmp / "flags.bin"), str(tmp / "grains.bin"), str(tmp / "out")], check=True)
+                    str(tmp / "flags.bin"), str(tmp / "grains.bin"), str(tmp / "out")], check=True, env=env)
     return {"dat": (tmp / "out_full.dat").read_bytes(), "grains": (tmp / "out.grains").read_bytes(),
             "vtk": (tmp / "out.vts").read_bytes()}
 
