@@ -95,6 +95,17 @@ typedef struct mpf_params {
     int32_t dg_range;              /* Range in cells; <= 0: (int)iwidth */
     double dg_threshold;           /* Threshold (PhiThreshold); <= 0: iwidth < 5 ? iwidth/15 : 1/3 */
     int32_t dg_weights;            /* WeightsMode (mpf_dg_weights) */
+    /* InterfaceRegularization (src/InterfaceRegularization.cpp): regularization = 1 makes every step
+     * DoubleObstacle::CalculatePhaseFieldIncrements(Phase, IP, Kappa) (src/DoubleObstacle.cpp:644-770:
+     * increments times RegularizationFactor, curvature (CurvatureFactor - RegularizationFactor) per
+     * pair), then Kappa.Average (:156-176) and Kappa.MergePhaseFieldIncrements (:403-456), before the
+     * driving force. ClearingMode AUTOMATIC. FP64 storage, single contexts (no adjoint, ensemble or
+     * mpf_step_into). */
+    int32_t regularization;
+    double curvature_factor;       /* InterfaceProperties CurvatureFactor (default 1.0) */
+    int32_t reg_average;           /* Average (default Yes): SetWeights + CollectAverage + DistributeAverage */
+    int32_t reg_range;             /* Range in cells; <= 0: (int)((iwidth + 1)/2) */
+    double reg_threshold;          /* Threshold (PhiThreshold); <= 0: DBL_EPSILON */
 } mpf_params;
 
 typedef struct mpf_ctx mpf_ctx;

@@ -59,6 +59,8 @@ class MpfParams(C.Structure):
         ("dg_limit", C.c_double), ("regularization_factor", C.c_double),
         ("dg_average", C.c_int32), ("dg_range", C.c_int32), ("dg_threshold", C.c_double),
         ("dg_weights", C.c_int32),
+        ("regularization", C.c_int32), ("curvature_factor", C.c_double),
+        ("reg_average", C.c_int32), ("reg_range", C.c_int32), ("reg_threshold", C.c_double),
     ]
 
 

@@ -46,6 +46,14 @@ class StepConfig:
     dg_threshold: float = 0.0
     dg_weights: int = 0
     regularization_factor: float = 1.0
+    # InterfaceRegularization: the steps take DoubleObstacle's curvature overload plus the module's
+    # Average and MergePhaseFieldIncrements; InterfaceProperties CurvatureFactor and the module's
+    # Average / Range / Threshold (Range <= 0: (IWidth + 1)/2; Threshold <= 0: DBL_EPSILON)
+    regularization: bool = False
+    curvature_factor: float = 1.0
+    reg_average: bool = True
+    reg_range: int = 0
+    reg_threshold: float = 0.0
     precision: int = _capi.MPF_FP64
     slots: int = 8
 
@@ -113,6 +121,11 @@ class Engine:
         p.dg_threshold = float(cfg.dg_threshold)
         p.dg_weights = int(cfg.dg_weights)
         p.regularization_factor = cfg.regularization_factor
+        p.regularization = int(cfg.regularization)
+        p.curvature_factor = float(cfg.curvature_factor)
+        p.reg_average = int(cfg.reg_average)
+        p.reg_range = int(cfg.reg_range)
+        p.reg_threshold = float(cfg.reg_threshold)
         h = C.c_void_p()
         st = self.lib.mpf_create(C.byref(g), C.byref(p), C.byref(h))
         if st != _capi.MPF_OK:

@@ -97,6 +97,13 @@ struct orc_ctx {
     long force_cap;
     /* per-step switches of the driving force (orc_set_step_terms) */
     int dg_step, avg_step;
+    /* InterfaceRegularization: curvature entries of the current step (CSR over cells, one per pair
+     * of non-zero entries of a wide-interface node, in the increments' pair order) and settings */
+    int reg_on, reg_active, reg_range;
+    double reg_threshold;
+    orc_force* curv;
+    long* curv_off;
+    long curv_cap;
     /* lists the last step started from (orc_keep_step_start) */
     int keep_ip, ip_valid;
     orc_node* ipnodes;
@@ -226,6 +233,20 @@ int orc_create(const orc_params* p, int32_t ngrains, const int32_t* grain_phase,
     }
     c->dg_step = 1;
     c->avg_step = 1;
+    /* InterfaceRegularization::Initialize / ReadInput (src/InterfaceRegularization.cpp:46-118) */
+    c->reg_on = p->regularization != 0;
+    c->reg_active = c->reg_on && (p->regularization_factor != 1.0 || p->curvature_factor != 1.0);
+    c->reg_range = p->reg_range > 0 ? p->reg_range : (int)((p->iwidth + 1) / 2);
+    c->reg_threshold = p->reg_threshold > 0.0 ? p->reg_threshold : DBL_EPSILON;
+    if (c->reg_on) {
+        if ((c->dn[0] && c->nx < c->reg_range) || (c->dn[1] && c->ny < c->reg_range) ||
+            (c->dn[2] && c->nz < c->reg_range)) {
+            free(c->force_off);
+            free(c);
+            return ORC_ERR_ARG;
+        }
+        c->curv_off = (long*)calloc((size_t)c->ncells + 1, sizeof(long));
+    }
     c->emodel = (int32_t*)calloc(np2, sizeof(int32_t));
     c->need_grad = 0;
     for (size_t i = 0; i < np2; ++i) {
@@ -245,7 +266,7 @@ int orc_create(const orc_params* p, int32_t ngrains, const int32_t* grain_phase,
     c->nodes = (orc_node*)calloc((size_t)c->ncells, sizeof(orc_node));
     c->tmp = (orc_node*)calloc((size_t)c->ncells, sizeof(orc_node));
     if (!c->sigma || !c->mu || !c->eps || !c->emodel || !c->gphase || !c->grot || !c->nodes || !c->tmp ||
-        (c->avg_on && !c->force_off)) {
+        (c->avg_on && !c->force_off) || (c->reg_on && !c->curv_off)) {
         orc_destroy(c);
         return ORC_ERR_NOMEM;
     }
@@ -276,6 +297,7 @@ void orc_destroy(orc_ctx* c)
     free(c->exist); free(c->stage);
     free(c->nodes); free(c->tmp);
     free(c->force); free(c->force_off); free(c->ipnodes);
+    free(c->curv); free(c->curv_off);
     free(c);
 }
 
@@ -571,16 +593,46 @@ typedef struct {
     int err;
 } local_result;
 
-static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
+/* loc_increment of pair (a, b) of a node (src/DoubleObstacle.cpp:517-562, the same terms at
+ * :702-752): the pair term, then per third entry the energy-difference term and the
+ * triple-junction term, before the mobility */
+static double loc_increment(const orc_ctx* c, const orc_node* nd, double sig[ORC_CAP][ORC_CAP], int a, int b,
+                            double scale, double prefactor)
 {
     const int n = nd->n;
-    double sig[ORC_CAP][ORC_CAP];
-    double mob[ORC_CAP][ORC_CAP];
-    memset(sig, 0, sizeof(sig));
-    memset(mob, 0, sizeof(mob));
+    const orc_entry* ea = &nd->e[a];
+    const orc_entry* eb = &nd->e[b];
+    double d = sig[a][b] * ((ea->lap + 0.5 * (1.0 + scale) * prefactor * ea->val) -
+                            (eb->lap + 0.5 * (1.0 + scale) * prefactor * eb->val));
+    if (n > 2)
+    for (int g = 0; g < n; ++g) {
+        if (g == a || g == b) continue;
+        const orc_entry* eg = &nd->e[g];
+        d += (sig[b][g] - sig[a][g]) * (eg->lap + 0.5 * (1.0 + scale) * prefactor * eg->val);
+        if (c->p.tj_model == ORC_TJ_MODEL) {
+            if (c->p.tj_factor != 0.0 || scale < 1.0) {
+                int pa = c->gphase[ea->idx], pb = c->gphase[eb->idx], pg = c->gphase[eg->idx];
+                int np = c->nphases;
+                d += std_max(c->p.tj_factor, (1.0 - scale)) * prefactor *
+                     (c->sigma[pa * np + pb] + c->sigma[pb * np + pg] + c->sigma[pa * np + pg]) *
+                     (ea->val * eg->val - eb->val * eg->val);
+            }
+        } else if (c->p.tj_model == ORC_TJ_VALUE) {
+            d += c->p.tj_energy * prefactor * (ea->val * eg->val - eb->val * eg->val);
+        }
+    }
+    return d;
+}
 
-    /* InterfaceProperties::SetSR (src/InterfaceProperties.cpp:603-762), ISO and CUBIC energy,
-     * ISO mobility; Properties(i,j,k) is cleared first so a missing pair reads 0. */
+/* InterfaceProperties::SetSR energies and mobilities of a node's pairs
+ * (src/InterfaceProperties.cpp:603-762): ISO and CUBIC energy, ISO mobility; a pair with both 0
+ * stays 0 (Properties(i,j,k) is cleared first). */
+static void pair_tables(const orc_ctx* c, const orc_node* nd, double sig[ORC_CAP][ORC_CAP],
+                        double mob[ORC_CAP][ORC_CAP], double* max_energy, double* max_mobility)
+{
+    const int n = nd->n;
+    memset(sig, 0, sizeof(double) * ORC_CAP * ORC_CAP);
+    memset(mob, 0, sizeof(double) * ORC_CAP * ORC_CAP);
     for (int a = 0; a < n; ++a)
     for (int b = a + 1; b < n; ++b) {
         const orc_entry* ea = &nd->e[a];
@@ -593,11 +645,21 @@ static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
             sig[a][b] = sig[b][a] = en;
             mob[a][b] = mob[b][a] = mo;
         }
-        res->max_energy = std_max(res->max_energy, en);
-        res->max_mobility = std_max(res->max_mobility, mo);
+        *max_energy = std_max(*max_energy, en);
+        *max_mobility = std_max(*max_mobility, mo);
     }
+}
 
-    /* DoubleObstacle::CalculatePhaseFieldIncrementsSR (src/DoubleObstacle.cpp:502-571) */
+static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
+{
+    const int n = nd->n;
+    double sig[ORC_CAP][ORC_CAP];
+    double mob[ORC_CAP][ORC_CAP];
+    pair_tables(c, nd, sig, mob, &res->max_energy, &res->max_mobility);
+
+    /* DoubleObstacle::CalculatePhaseFieldIncrementsSR (src/DoubleObstacle.cpp:502-571), or with
+     * InterfaceRegularization the curvature overload (:686-770): the same loc_increment, scaled left
+     * to right (loc_increment * mobility * norm_1 * scale) and then by RegularizationFactor */
     orc_pair dot[ORC_PCAP];
     int nd_dot = 0;
     const double eta = c->p.iwidth * c->p.dx;
@@ -614,27 +676,33 @@ static void local_step(orc_ctx* c, orc_node* nd, double dt, local_result* res)
         const orc_entry* ea = &nd->e[a];
         const orc_entry* eb = &nd->e[b];
         double scale = sqrt(c->vr[ea->idx] * c->vr[eb->idx]);
-        double d = sig[a][b] * ((ea->lap + 0.5 * (1.0 + scale) * prefactor * ea->val) -
-                                (eb->lap + 0.5 * (1.0 + scale) * prefactor * eb->val));
-        if (n > 2)
-        for (int g = 0; g < n; ++g) {
-            if (g == a || g == b) continue;
-            const orc_entry* eg = &nd->e[g];
-            d += (sig[b][g] - sig[a][g]) * (eg->lap + 0.5 * (1.0 + scale) * prefactor * eg->val);
-            if (c->p.tj_model == ORC_TJ_MODEL) {
-                if (c->p.tj_factor != 0.0 || scale < 1.0) {
-                    int pa = c->gphase[ea->idx], pb = c->gphase[eb->idx], pg = c->gphase[eg->idx];
-                    int np = c->nphases;
-                    d += std_max(c->p.tj_factor, (1.0 - scale)) * prefactor *
-                         (c->sigma[pa * np + pb] + c->sigma[pb * np + pg] + c->sigma[pa * np + pg]) *
-                         (ea->val * eg->val - eb->val * eg->val);
-                }
-            } else if (c->p.tj_model == ORC_TJ_VALUE) {
-                d += c->p.tj_energy * prefactor * (ea->val * eg->val - eb->val * eg->val);
-            }
+        double d = loc_increment(c, nd, sig, a, b, scale, prefactor);
+        if (c->reg_on) {
+            d = d * mob[a][b] * norm_1 * scale;
+            if (c->reg_active) d *= c->p.regularization_factor;
+        } else {
+            d *= mob[a][b] * norm_1 * scale;
         }
-        d *= mob[a][b] * norm_1 * scale;
         if (ab_add_asym1(dot, &nd_dot, ea->idx, eb->idx, d)) { res->err = ORC_ERR_CAPACITY; return; }
+    }
+
+    /* InterfaceRegularization::MergePhaseFieldIncrementsSR (src/InterfaceRegularization.cpp:425-456)
+     * on interface nodes: (average or raw) * mobility * weight * Pi/Eta per curvature entry. The
+     * weight is 0 unless Average (SetWeightsSR) ran this step. */
+    if (c->reg_active && nd->flag == 2) {
+        const double pref = ORC_PI / eta;
+        const long ci = nd - c->nodes;
+        for (long q = c->curv_off[ci]; q < c->curv_off[ci + 1]; ++q) {
+            const orc_force* f = &c->curv[q];
+            int ia = -1, ib = -1;
+            for (int t = 0; t < n; ++t) {
+                if (nd->e[t].idx == f->a) ia = t;
+                if (nd->e[t].idx == f->b) ib = t;
+            }
+            const double kappa = c->p.reg_average ? f->average : f->raw;
+            const double dpsi = kappa * mob[ia][ib] * f->weight * pref;
+            if (ab_add_asym1(dot, &nd_dot, f->a, f->b, dpsi)) { res->err = ORC_ERR_CAPACITY; return; }
+        }
     }
 
     /* UserDrivingForce::SetDrivingForce (src/UserDrivingForce.cpp:152-175) on interface nodes,
@@ -960,6 +1028,139 @@ static void dg_average_pass(orc_ctx* c, int second)
     }
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Interface regularization (curvature averaged over the interface)                             */
+
+/* The curvature part of DoubleObstacle::CalculatePhaseFieldIncrementsSR with InterfaceRegularization
+ * (src/DoubleObstacle.cpp:686-770) after InterfaceRegularization::Clear: on every wide-interface node,
+ * per pair of its list with both values non-zero (pair order a < b),
+ *   raw = loc_increment * Eta/Pi * scale * norm_1 / sqrt(phi_a phi_b) * (CurvatureFactor - RegularizationFactor)
+ * (NodeDF::add_raw on a fresh entry); then SetWeightsSR (src/InterfaceRegularization.cpp:178-194) on
+ * interface nodes, weight = sqrt(phi_a phi_b), when Average is on. */
+static int curv_set_raw(orc_ctx* c)
+{
+    const long N = c->ncells;
+    c->curv_off[0] = 0;
+    for (long ci = 0; ci < N; ++ci) {
+        const orc_node* nd = &c->nodes[ci];
+        long m = 0;
+        if (nd->flag != 0)
+            for (int a = 0; a < nd->n; ++a)
+            for (int b = a + 1; b < nd->n; ++b)
+                m += nd->e[a].val != 0.0 && nd->e[b].val != 0.0;
+        c->curv_off[ci + 1] = c->curv_off[ci] + m;
+    }
+    long total = c->curv_off[N];
+    if (total > c->curv_cap) {
+        orc_force* f = (orc_force*)realloc(c->curv, (size_t)(total ? total : 1) * sizeof(orc_force));
+        if (!f) return ORC_ERR_NOMEM;
+        c->curv = f;
+        c->curv_cap = total;
+    }
+    const double eta = c->p.iwidth * c->p.dx;
+    const double prefactor = ORC_PI * ORC_PI / (eta * eta);
+    const double prefactor2 = eta / ORC_PI;
+#pragma omp parallel for schedule(static)
+    for (long ci = 0; ci < N; ++ci) {
+        const orc_node* nd = &c->nodes[ci];
+        if (nd->flag == 0) continue;
+        orc_force* f = &c->curv[c->curv_off[ci]];
+        double sig[ORC_CAP][ORC_CAP], mob[ORC_CAP][ORC_CAP], me = 0.0, mm = 0.0;
+        pair_tables(c, nd, sig, mob, &me, &mm);
+        double localn = 0.0;
+        if (c->p.consider_nucleus_volume) {
+            for (int a = 0; a < nd->n; ++a) localn += c->vr[nd->e[a].idx];
+        } else {
+            localn = (double)nd->n;
+        }
+        const double norm_1 = 1.0 / localn;
+        for (int a = 0; a < nd->n; ++a)
+        for (int b = a + 1; b < nd->n; ++b) {
+            const orc_entry* ea = &nd->e[a];
+            const orc_entry* eb = &nd->e[b];
+            if (!(ea->val != 0.0 && eb->val != 0.0)) continue;
+            const double scale = sqrt(c->vr[ea->idx] * c->vr[eb->idx]);
+            const double li = loc_increment(c, nd, sig, a, b, scale, prefactor);
+            double kappa = li * prefactor2 * scale * norm_1 / sqrt(ea->val * eb->val);
+            kappa *= c->p.curvature_factor - c->p.regularization_factor;
+            f->a = ea->idx; f->b = eb->idx;
+            f->raw = kappa;
+            f->tmp = 0.0; f->average = 0.0;
+            f->weight = (c->p.reg_average && nd->flag == 2) ? sqrt(ea->val * eb->val) : 0.0;
+            ++f;
+        }
+    }
+    return ORC_OK;
+}
+
+static const orc_force* curv_find(const orc_ctx* c, long cell, uint32_t a, uint32_t b, double* sign)
+{
+    for (long q = c->curv_off[cell]; q < c->curv_off[cell + 1]; ++q) {
+        const orc_force* f = &c->curv[q];
+        if (f->a == a && f->b == b) { *sign = 1.0; return f; }
+        if (f->a == b && f->b == a) { *sign = -1.0; return f; }
+    }
+    return NULL;
+}
+
+/* InterfaceRegularization::CollectAverageSR (second == 0, src/InterfaceRegularization.cpp:196-223)
+ * and DistributeAverageSR (second == 1, :225-274) on interface nodes: over the sphere of radius
+ * Range (offsets x outer, z inner, each clipped to N-1 on its axis; interface neighbours only),
+ * tmp = sum(weight*(Range-d)*raw)/sum(weight*(Range-d)) over terms above DBL_EPSILON, for entries
+ * whose weight exceeds sqrt(T(1-T)); average = the mean of the neighbours' tmp whose weight exceeds
+ * sqrt(T(1-T)), for entries whose weight exceeds DBL_EPSILON. Seed grains keep raw. The periodic
+ * BoundaryConditions copies of Curvature between the passes are wrapped indices here. */
+static void curv_average_pass(orc_ctx* c, int second)
+{
+    const int R = c->reg_range;
+    const int xr = (R < c->nx - 1 ? R : c->nx - 1) * c->dn[0];
+    const int yr = (R < c->ny - 1 ? R : c->ny - 1) * c->dn[1];
+    const int zr = (R < c->nz - 1 ? R : c->nz - 1) * c->dn[2];
+    const double thr = sqrt(c->reg_threshold * (1.0 - c->reg_threshold));
+    const long N = c->ncells;
+#pragma omp parallel for schedule(static)
+    for (long ci = 0; ci < N; ++ci) {
+        if (c->nodes[ci].flag != 2) continue;
+        long i = ci / ((long)c->ny * c->nz), j = (ci / c->nz) % c->ny, k = ci % c->nz;
+        for (long q = c->curv_off[ci]; q < c->curv_off[ci + 1]; ++q) {
+            orc_force* e = &c->curv[q];
+            if (second ? !(e->weight > DBL_EPSILON) : !(e->weight > thr)) {
+                if (c->stage[e->a] == 0 || c->stage[e->b] == 0) {
+                    if (second) e->average = e->raw;
+                    else e->tmp = e->raw;
+                }
+                continue;
+            }
+            double value = 0.0, sum = 0.0;
+            int counter = 0;
+            for (int ii = -xr; ii <= xr; ii++)
+            for (int jj = -yr; jj <= yr; jj++)
+            for (int kk = -zr; kk <= zr; kk++) {
+                double wdist = R - sqrt((double)(ii * ii + jj * jj + kk * kk));
+                long nb = cell_index(c, wrap(i + ii, c->nx), wrap(j + jj, c->ny), wrap(k + kk, c->nz));
+                if (!(wdist > 0.0) || c->nodes[nb].flag != 2) continue;
+                double sign = 1.0;
+                const orc_force* f = curv_find(c, nb, e->a, e->b, &sign);
+                const double wphi = f ? f->weight : 0.0;
+                if (second) {
+                    if (wphi > thr) {
+                        counter++;
+                        value += f->tmp * sign;
+                    }
+                } else {
+                    const double w = wphi * wdist;
+                    if (w > DBL_EPSILON) {
+                        sum += w;
+                        value += w * (f->raw * sign);
+                    }
+                }
+            }
+            if (second) e->average = counter != 0 ? value / counter : e->tmp;
+            else if (sum > DBL_EPSILON) e->tmp = value / sum;
+        }
+    }
+}
+
 int orc_step(orc_ctx* c, double dt, int32_t nsteps)
 {
     for (int s = 0; s < nsteps; ++s) {
@@ -974,6 +1175,16 @@ int orc_step(orc_ctx* c, double dt, int32_t nsteps)
             if (st) return st;
             dg_average_pass(c, 0);
             dg_average_pass(c, 1);
+        }
+        if (c->reg_active) {
+            /* DO.CalculatePhaseFieldIncrements(Phase, IP, Kappa)'s curvature, Kappa.Average
+             * (src/InterfaceRegularization.cpp:156-176), on the step-start lists */
+            int st = curv_set_raw(c);
+            if (st) return st;
+            if (c->p.reg_average) {
+                curv_average_pass(c, 0);
+                curv_average_pass(c, 1);
+            }
         }
         double mx_e = 0.0, mx_m = 0.0;
         int64_t warn = 0;

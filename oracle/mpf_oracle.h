@@ -57,6 +57,13 @@ typedef struct orc_params {
     int32_t dg_range;               /* Range in cells; <= 0: (int)iwidth (DrivingForce.cpp:64) */
     double dg_threshold;            /* PhiThreshold; <= 0: iwidth < 5 ? iwidth/15 : 1/3 (:66-73) */
     int32_t dg_weights;             /* WeightsMode: 0 PHASEFIELDS, 1 RANGE, 2 COUNTER (:135-152) */
+    /* InterfaceRegularization: 1 = the steps use DoubleObstacle's curvature overload
+     * (src/DoubleObstacle.cpp:686-770) + InterfaceRegularization::Average / MergePhaseFieldIncrements */
+    int32_t regularization;
+    double curvature_factor;        /* InterfaceProperties::CurvatureFactor (1.0) */
+    int32_t reg_average;            /* Average (Yes) */
+    int32_t reg_range;              /* Range; <= 0: (int)((iwidth + 1)/2) (InterfaceRegularization.cpp:61) */
+    double reg_threshold;           /* Threshold; <= 0: DBL_EPSILON (:59) */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

@@ -30,6 +30,8 @@ class OrcParams(C.Structure):
         ("dg_limit", C.c_double), ("regularization_factor", C.c_double),
         ("dg_average", C.c_int32), ("dg_range", C.c_int32), ("dg_threshold", C.c_double),
         ("dg_weights", C.c_int32),
+        ("regularization", C.c_int32), ("curvature_factor", C.c_double),
+        ("reg_average", C.c_int32), ("reg_range", C.c_int32), ("reg_threshold", C.c_double),
     ]
 
 
@@ -112,6 +114,11 @@ class Oracle:
         p.dg_range = int(cfg.dg_range)
         p.dg_threshold = float(cfg.dg_threshold)
         p.dg_weights = int(cfg.dg_weights)
+        p.regularization = int(getattr(cfg, "regularization", False))
+        p.curvature_factor = float(getattr(cfg, "curvature_factor", 1.0))
+        p.reg_average = int(getattr(cfg, "reg_average", True))
+        p.reg_range = int(getattr(cfg, "reg_range", 0))
+        p.reg_threshold = float(getattr(cfg, "reg_threshold", 0.0))
         self._p = p
         self.h = None
         self.ngrains = 0
@@ -288,6 +295,7 @@ def write_opi(path, cfg, threads: int = 1, dt: float | None = None, nsteps: int 
     if tj == "VALUE":
         lines.append(f"$TripleJunctionEnergy : {cfg.tj_energy!r}")
     lines.append(f"$RegularizationFactor : {cfg.regularization_factor!r}")
+    lines.append(f"$CurvatureFactor : {cfg.curvature_factor!r}")
     lines += ["", "@BoundaryConditions"]
     for ax in "XYZ":
         lines += [f"$BC0{ax} : Periodic", f"$BCN{ax} : Periodic"]
@@ -309,6 +317,12 @@ def write_opi(path, cfg, threads: int = 1, dt: float | None = None, nsteps: int 
         for a in range(cfg.nphases):
             for b in range(a, cfg.nphases):
                 lines.append(f"$Limit_{a}_{b} : {cfg.dg_limit!r}")
+    if cfg.regularization:
+        lines += ["", "@InterfaceRegularization", f"$Average : {'Yes' if cfg.reg_average else 'No'}"]
+        if cfg.reg_range > 0:
+            lines.append(f"$Range : {int(cfg.reg_range)}")
+        if cfg.reg_threshold > 0.0:
+            lines.append(f"$Threshold : {float(cfg.reg_threshold)!r}")
     Path(path).write_text("\n".join(lines) + "\n")
 
 
@@ -335,6 +349,8 @@ def run_reference(workdir, cfg, lists, grain_phase, quaternions, nsteps, dt, dum
            repr(float(dt)), str(workdir / "out"), ",".join(str(s) for s in dumps), "--threads", str(threads)]
     if cfg.dg_mode is not None:
         cmd.append("--driving-force")
+    if cfg.regularization:
+        cmd.append("--regularization")
     if untimed:
         cmd += ["--untimed", str(int(untimed))]
     if energy:

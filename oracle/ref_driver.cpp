@@ -31,6 +31,10 @@
 // more IP.Set on the final state ("energy_ip_current").
 // --dg-every N: with --driving-force, UserDrivingForce::SetDrivingForce only in steps divisible
 // by N (DrivingForce::Clear, Average and MergePhaseFieldIncrements in every step).
+// --regularization: the InterfaceRegularization overload of the step
+// (src/DoubleObstacle.cpp:644-684, 686-770): DO.CalculatePhaseFieldIncrements(Phi, IP[, dG], Kappa),
+// then Kappa.Average(Phi, BC) and Kappa.MergePhaseFieldIncrements(Phi, IP)
+// (src/InterfaceRegularization.cpp:156-176, 403-456), before the driving-force calls.
 
 #include "Settings.h"
 #include "RunTimeControl.h"
@@ -40,6 +44,7 @@
 #include "InterfaceProperties.h"
 #include "DrivingForce.h"
 #include "UserDrivingForce.h"
+#include "InterfaceRegularization.h"
 
 #include <cstdio>
 #include <cstdint>
@@ -101,7 +106,7 @@ int main(int argc, char* argv[])
         std::string tok;
         while (std::getline(ss, tok, ',')) if (!tok.empty()) dumps.insert(std::atoi(tok.c_str()));
     }
-    bool with_dg = false, write_files = false, energy = false;
+    bool with_dg = false, write_files = false, energy = false, with_reg = false;
     std::string normalize_only;
     int threads = 1, untimed = 0, dg_every = 1;
     for (int a = 8; a < argc; ++a)
@@ -113,6 +118,7 @@ int main(int argc, char* argv[])
         if (s == "--normalize-only" && a + 1 < argc) normalize_only = argv[++a];
         if (s == "--untimed" && a + 1 < argc) untimed = std::atoi(argv[++a]);
         if (s == "--energy") energy = true;
+        if (s == "--regularization") with_reg = true;
         if (s == "--dg-every" && a + 1 < argc) dg_every = std::atoi(argv[++a]);
     }
     omp_set_num_threads(threads);
@@ -130,6 +136,7 @@ int main(int argc, char* argv[])
         dG = new DrivingForce(OPSettings, opi);
         UDF = new UserDrivingForce(OPSettings, opi);
     }
+    InterfaceRegularization* Kappa = with_reg ? new InterfaceRegularization(OPSettings, opi) : nullptr;
 
     std::ifstream gin(grains, std::ios::binary);
     uint32_t ng = 0;
@@ -201,8 +208,15 @@ int main(int argc, char* argv[])
         if (with_dg) dG->Clear();
         IP.Set(Phi, BC);
         double t1 = omp_get_wtime();
-        if (with_dg) DO.CalculatePhaseFieldIncrements(Phi, IP, *dG);
-        else         DO.CalculatePhaseFieldIncrements(Phi, IP);
+        if (with_reg)
+        {
+            if (with_dg) DO.CalculatePhaseFieldIncrements(Phi, IP, *dG, *Kappa);
+            else         DO.CalculatePhaseFieldIncrements(Phi, IP, *Kappa);
+            Kappa->Average(Phi, BC);
+            Kappa->MergePhaseFieldIncrements(Phi, IP);
+        }
+        else if (with_dg) DO.CalculatePhaseFieldIncrements(Phi, IP, *dG);
+        else              DO.CalculatePhaseFieldIncrements(Phi, IP);
         double t2 = omp_get_wtime();
         if (with_dg)
         {

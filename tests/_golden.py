@@ -30,6 +30,12 @@ class Case:
             dg_range=int(z["dg_range"]) if "dg_range" in z else 0,
             dg_threshold=float(z["dg_threshold"]) if "dg_threshold" in z else 0.0,
             dg_weights=int(z["dg_weights"]) if "dg_weights" in z else 0,
+            regularization_factor=float(z["regularization_factor"]) if "regularization_factor" in z else 1.0,
+            regularization=bool(z["regularization"]) if "regularization" in z else False,
+            curvature_factor=float(z["curvature_factor"]) if "curvature_factor" in z else 1.0,
+            reg_average=bool(z["reg_average"]) if "reg_average" in z else True,
+            reg_range=int(z["reg_range"]) if "reg_range" in z else 0,
+            reg_threshold=float(z["reg_threshold"]) if "reg_threshold" in z else 0.0,
             precision=precision,
             slots=16,  # the reference's lists are unbounded; the densest fixture needs 9
         )

@@ -74,6 +74,27 @@ def cases():
     # the averaged three-phase driving force set only in every third step (the driver skips
     # UserDrivingForce::SetDrivingForce in the others)
     out.append(("dg_every3_2d_32", three, 12, 23, phase3, None, 12, [0, 1, 3, 7, 12], 0.9, 3))
+    # interface regularization (DoubleObstacle's curvature overload + InterfaceRegularization
+    # Average / MergePhaseFieldIncrements): 2D with the default Range / Threshold and small grains
+    # (VolumeRatio < 1); 3D with two phases, TJ VALUE, Range 2, Threshold 0.05; curvature merged
+    # unaveraged (Average: No, so the merge adds weight 0); the factors at 1 (the overload's own
+    # rounding); and the regularized step with the averaged driving force
+    reg2 = StepConfig(32, 32, 0, regularization=True, regularization_factor=1.5, curvature_factor=0.7)
+    out.append(("reg2d_32", reg2, 14, 31, None, None, 30, [0, 1, 10, 30], 0.9))
+    reg3 = StepConfig(16, 16, 16, nphases=2, sigma=np.array([[0.24, 0.30], [0.30, 0.20]]),
+                      mu=np.array([[4e-9, 2e-9], [2e-9, 3e-9]]), tj_model=1, tj_energy=0.05,
+                      regularization=True, regularization_factor=2.0, curvature_factor=1.0,
+                      reg_range=2, reg_threshold=0.05)
+    out.append(("reg3d_16", reg3, 10, 32, (np.arange(10) % 2).astype(np.int32), None, 12, [0, 1, 6, 12], 0.9))
+    regn = StepConfig(24, 24, 0, regularization=True, regularization_factor=0.8, curvature_factor=0.5,
+                      reg_average=False)
+    out.append(("reg_noavg2d_24", regn, 20, 33, None, None, 20, [0, 1, 20], 1.0))
+    reg1 = StepConfig(24, 24, 0, regularization=True)
+    out.append(("reg_unit2d_24", reg1, 10, 34, None, None, 20, [0, 1, 20], 1.0))
+    regdg = StepConfig(32, 32, 0, nphases=3, sigma=np.full((3, 3), 0.24), mu=np.full((3, 3), 4e-9),
+                       dg_mode=np.ones((3, 3), np.int32) - np.eye(3, dtype=np.int32), dg_value=dg3,
+                       dg_average=True, regularization=True, regularization_factor=1.2, curvature_factor=0.6)
+    out.append(("reg_dg2d_32", regdg, 12, 35, phase3, None, 20, [0, 1, 10, 20], 0.9))
     return out
 
 
@@ -149,6 +170,9 @@ def main(only=()):
             # the final state; the step period of UserDrivingForce::SetDrivingForce
             energy=np.asarray(timing["energy"], np.float64), energy_ip_current=float(timing["energy_ip_current"]),
             dg_every=int(dg_every),
+            regularization_factor=float(cfg.regularization_factor), regularization=int(cfg.regularization),
+            curvature_factor=float(cfg.curvature_factor), reg_average=int(cfg.reg_average),
+            reg_range=int(cfg.reg_range), reg_threshold=float(cfg.reg_threshold),
         )
         for s in dumps:
             st, vol, vr = ref[s]

@@ -273,3 +273,48 @@ def test_averaged_constant_driving_force_is_the_constant():
     assert np.abs(lb.value - lc.value).max() < 1e-13
     for o in (b, c):
         o.close()
+
+
+@pytest.mark.parametrize("name,change", [
+    ("reg2d_32", {"regularization": False}), ("reg2d_32", {"reg_average": False}), ("reg2d_32", {"reg_range": 2}),
+    ("reg3d_16", {"reg_threshold": 0.0}), ("reg_noavg2d_24", {"reg_average": True}),
+    ("reg_unit2d_24", {"regularization": False}), ("reg_dg2d_32", {"reg_average": False}),
+])
+def test_golden_pins_interface_regularization(name, change):
+    """The regularization fixtures separate the curvature overload from the plain step, averaged
+    from unaveraged curvature, and the Range / Threshold settings; with both factors at 1 they still
+    hold the overload's own rounding (loc_increment * mobility * norm_1 * scale, left to right)."""
+    import dataclasses
+
+    from _golden import Case
+
+    case = Case(name)
+    orc = case.setup(O.Oracle(dataclasses.replace(case.cfg, **change)))
+    differs = False
+    for s, got, _, _ in case.run(orc):
+        exp = case.expect[s][0]
+        differs = differs or not (np.array_equal(got.count, exp.count) and np.array_equal(got.index, exp.index)
+                                  and np.array_equal(got.value, exp.value))
+    orc.close()
+    assert differs
+
+
+def test_regularization_with_equal_factors_doubles_the_mobility():
+    """Closed form: with CurvatureFactor = RegularizationFactor = 2 the curvature vanishes
+    ((CF - RF) = 0) and the increments are twice the plain ones, i.e. the plain step with twice the
+    mobility, to rounding (a factor applied twice, or in the wrong place, moves values by O(dt))."""
+    import dataclasses
+
+    from _golden import Case
+
+    case = Case("reg2d_32")
+    a = case.setup(O.Oracle(dataclasses.replace(case.cfg, regularization_factor=2.0, curvature_factor=2.0)))
+    b = case.setup(O.Oracle(dataclasses.replace(case.cfg, regularization=False, mu=2.0 * np.asarray(case.cfg.mu))))
+    dt = 0.5 * case.dt
+    for o in (a, b):
+        o.step(dt, 5)
+    la, lb = a.download(), b.download()
+    assert np.array_equal(la.count, lb.count) and np.array_equal(la.index, lb.index)
+    assert np.abs(la.value - lb.value).max() < 1e-13
+    for o in (a, b):
+        o.close()
