@@ -69,6 +69,18 @@ struct mpf_ctx {
     double* d_avg = nullptr;
     int32_t* d_avg_off = nullptr;
     double* d_avg_off_w = nullptr;
+    // interface regularization (InterfaceRegularization): raw, tmp and averaged curvature per pair
+    // of own entries, and the averaging sphere
+    bool reg = false, reg_active = false, reg_average = true;
+    double curvature_factor = 1.0, reg_threshold = 0.0;
+    int reg_range = 0;
+    std::vector<int32_t> reg_off;
+    std::vector<double> reg_off_w;
+    double* d_curv_raw = nullptr;
+    double* d_curv_tmp = nullptr;
+    double* d_curv = nullptr;
+    int32_t* d_reg_off = nullptr;
+    double* d_reg_off_w = nullptr;
     // per-step terms (mpf_set_step_terms): the driving force acts / is averaged in the next steps
     bool dg_on = true, avg_on = true;
     int32_t* d_dgmode_off = nullptr;  // all-zero mode table: no pair takes the driving force
@@ -194,7 +206,7 @@ bool dg_active(const mpf_ctx* c) { return c->dg && c->dg_on; }
 // DrivingForce::Average averages it over the interface in the next step.
 bool avg_active(const mpf_ctx* c) { return c->dg_avg && c->dg_on && c->avg_on; }
 // Kernel variant of the next step: orientation-dependent energies and/or an acting driving force.
-bool full_variant(const mpf_ctx* c) { return c->grad || dg_active(c); }
+bool full_variant(const mpf_ctx* c) { return c->grad || dg_active(c) || c->reg; }
 
 void free_grains(mpf_ctx* c)
 {
@@ -226,6 +238,10 @@ mpf::StepArgs step_args(const mpf_ctx* c, double dt, int src)
     a.emodel = c->d_emodel; a.dg_value = c->d_dgval; a.dg_allowed = c->d_dgallowed;
     a.dg_mode = dg_active(c) ? c->d_dgmode : c->d_dgmode_off;
     a.dg_avg = avg_active(c) ? c->d_avg : nullptr;
+    a.reg = c->reg ? 1 : 0;
+    a.reg_active = c->reg_active ? 1 : 0;
+    a.reg_factor = c->regularization;
+    a.curv = c->reg_active && c->reg_average ? c->d_curv : nullptr;
     a.gphase = c->g_phase; a.gvr = c->g_vr; a.grot = c->g_rot; a.gsolid = c->g_solid;
     a.cnt_in = c->cnt[src]; a.id_in = c->id[src]; a.val_in = c->val[src];
     a.cnt_out = c->cnt[1 - src]; a.id_out = c->id[1 - src]; a.val_out = c->val[1 - src];
@@ -347,11 +363,33 @@ mpf::AvgArgs avg_args(const mpf_ctx* c, const mpf::StepArgs& s)
     return a;
 }
 
+mpf::CurvArgs curv_args(const mpf_ctx* c, const mpf::StepArgs& s)
+{
+    mpf::CurvArgs a{};
+    a.s = s;
+    a.raw = c->d_curv_raw;
+    a.tmp = c->d_curv_tmp;
+    a.avg = c->d_curv;
+    a.off = c->d_reg_off;
+    a.off_w = c->d_reg_off_w;
+    a.noff = static_cast<int32_t>(c->reg_off_w.size());
+    a.threshold = std::sqrt(c->reg_threshold * (1.0 - c->reg_threshold));
+    a.factor = c->curvature_factor - c->regularization;
+    a.prefactor2 = c->grid.iwidth * c->grid.dx / kPi;
+    return a;
+}
+
 mpf_status launch_step_part(mpf_ctx* c, hipStream_t stream)
 {
     if (avg_active(c)) {
         mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_dg_average(avg_args(c, c->h_args[c->cur]), c->fp32, stream)),
                                "driving-force averaging kernels");
+        if (st) return st;
+    }
+    if (c->reg_active && c->reg_average) {
+        // the curvature only enters through its average: unaveraged, its merge weight is 0
+        mpf_status st = c->hip(static_cast<hipError_t>(mpf::launch_curvature(curv_args(c, c->h_args[c->cur]), true, stream)),
+                               "interface regularization kernels");
         if (st) return st;
     }
     return c->hip(launch_main(c, c->h_args[c->cur], stream), "step kernel");
@@ -461,6 +499,31 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     c->consider_nucleus = params->consider_nucleus_volume;
     c->nucleus_volume_factor = params->nucleus_volume_factor;
     c->regularization = params->regularization_factor;
+    // InterfaceRegularization::Initialize / ReadInput (src/InterfaceRegularization.cpp:46-118)
+    c->reg = params->regularization != 0;
+    if (c->reg) {
+        if (c->fp32) return bail(MPF_ERR_UNSUPPORTED, "interface regularization needs FP64 storage");
+        c->curvature_factor = params->curvature_factor;
+        c->reg_active = params->regularization_factor != 1.0 || params->curvature_factor != 1.0;
+        c->reg_average = params->reg_average != 0;
+        c->reg_range = params->reg_range > 0 ? params->reg_range : static_cast<int>((grid->iwidth + 1) / 2);
+        c->reg_threshold = params->reg_threshold > 0.0 ? params->reg_threshold : DBL_EPSILON;
+        c->uniform = false;
+        if ((c->dnx && c->nx < c->reg_range) || (c->dny && c->ny < c->reg_range) || (c->dnz && c->nz < c->reg_range))
+            return bail(MPF_ERR_UNSUPPORTED, "interface regularization needs every active axis >= Range cells");
+        // CollectAverageSR / DistributeAverageSR loop bounds and distance weights (:198-213)
+        const int xr = std::min(c->reg_range, c->nx - 1) * c->dnx;
+        const int yr = std::min(c->reg_range, c->ny - 1) * c->dny;
+        const int zr = std::min(c->reg_range, c->nz - 1) * c->dnz;
+        for (int ii = -xr; ii <= xr; ii++)
+            for (int jj = -yr; jj <= yr; jj++)
+                for (int kk = -zr; kk <= zr; kk++) {
+                    const double wd = c->reg_range - std::sqrt(static_cast<double>(ii * ii + jj * jj + kk * kk));
+                    if (!(wd > 0.0)) continue;
+                    c->reg_off.insert(c->reg_off.end(), {ii, jj, kk});
+                    c->reg_off_w.push_back(wd);
+                }
+    }
 
     const double eta = grid->iwidth * grid->dx;
     c->prefactor = kPi * kPi / (eta * eta);
@@ -548,6 +611,15 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
         c->hip(dalloc(&c->d_work_list, 4 * N), "work list") || c->hip(dalloc(&c->d_work_count, 4), "work list") ||
         c->hip(dalloc(&c->d_stats, 2), "statistics") || c->hip(dalloc(&c->d_obs, 4), "statistics"))
         return bail(MPF_ERR_NOMEM, c->err.c_str());
+    if (c->reg_active && c->reg_average) {
+        const size_t pairs = static_cast<size_t>(c->slots) * (c->slots - 1) / 2;
+        if (c->hip(dalloc(&c->d_curv_raw, pairs * N), "regularization arrays") ||
+            c->hip(dalloc(&c->d_curv_tmp, pairs * N), "regularization arrays") ||
+            c->hip(dalloc(&c->d_curv, pairs * N), "regularization arrays") ||
+            c->hip(dalloc(&c->d_reg_off, c->reg_off.size()), "regularization arrays") ||
+            c->hip(dalloc(&c->d_reg_off_w, c->reg_off_w.size()), "regularization arrays"))
+            return bail(MPF_ERR_NOMEM, c->err.c_str());
+    }
     if (c->dg_avg) {
         const size_t pairs = static_cast<size_t>(c->slots) * (c->slots - 1) / 2;
         if (c->hip(dalloc(&c->d_avg_tmp, pairs * N), "averaging arrays") || c->hip(dalloc(&c->d_avg, pairs * N), "averaging arrays") ||
@@ -566,6 +638,10 @@ mpf_status mpf_create(const mpf_grid* grid, const mpf_params* params, mpf_ctx** 
     h2d(e, c->d_dgmode_off, no_mode.data(), np2);
     h2d(e, c->d_dgval, c->dgval.data(), np2);
     h2d(e, c->d_dgallowed, c->dgallowed.data(), np2);
+    if (c->reg_active && c->reg_average) {
+        h2d(e, c->d_reg_off, c->reg_off.data(), c->reg_off.size());
+        h2d(e, c->d_reg_off_w, c->reg_off_w.data(), c->reg_off_w.size());
+    }
     if (c->dg_avg) {
         h2d(e, c->d_avg_off, c->avg_off.data(), c->avg_off.size());
         h2d(e, c->d_avg_off_w, c->avg_off_w.data(), c->avg_off_w.size());
@@ -1071,6 +1147,7 @@ void mpf_destroy(mpf_ctx* c)
     dfree(c->d_status); dfree(c->d_first_overflow); dfree(c->d_warnings);
     dfree(c->d_work_list); dfree(c->d_work_count); dfree(c->d_stats); dfree(c->d_obs);
     dfree(c->d_avg_tmp); dfree(c->d_avg); dfree(c->d_avg_off); dfree(c->d_avg_off_w);
+    dfree(c->d_curv_raw); dfree(c->d_curv_tmp); dfree(c->d_curv); dfree(c->d_reg_off); dfree(c->d_reg_off_w);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1088,7 +1165,7 @@ mpf_status mpf_set_pair_params(mpf_ctx* c, const double* sigma, const double* mu
     // derived tables as in mpf_create: MaxEnergy, the tanh amplitude, the equal-energy flag
     for (size_t i = 0; i < np2; ++i)
         c->maxe[i] = c->emodel[i] == MPF_ENERGY_CUBIC ? c->sigma[i] * (1.0 + c->eps[i] * 1.41) : c->sigma[i];
-    c->uniform = !c->grad && ((c->tj_model == MPF_TJ_MODEL && c->tj_factor == 0.0) || c->tj_model == MPF_TJ_NONE);
+    c->uniform = !c->grad && !c->reg && ((c->tj_model == MPF_TJ_MODEL && c->tj_factor == 0.0) || c->tj_model == MPF_TJ_NONE);
     for (size_t i = 0; i < np2; ++i)
         if (c->sigma[i] != c->sigma[0] || c->mu[i] != c->mu[0]) c->uniform = false;
     if (c->dg)
@@ -1197,6 +1274,7 @@ mpf_status mpf_adjoint_step(mpf_ctx* c, double dt, const double* lam_next, doubl
     if ((next_count == nullptr) != (next_index == nullptr)) return c->fail(MPF_ERR_ARG, "give both next_count and next_index");
     if (c->fp32 && !next_count) return c->fail(MPF_ERR_ARG, "FP32 states need the next state's lists (next_count, next_index)");
     if (avg_active(c)) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint takes the driving force without averaging");
+    if (c->reg) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint does not differentiate interface regularization");
     if (c->nphases > mpf::kAdjPhases) return c->fail(MPF_ERR_UNSUPPORTED, "the adjoint handles at most 4 phases");
     const size_t N = static_cast<size_t>(c->ncells);
     constexpr size_t KA = mpf::kAdjSlots;
@@ -1568,6 +1646,7 @@ mpf_status mpf_ensemble_create(mpf_ctx* const* members, int32_t n, mpf_ensemble*
     *out = nullptr;
     for (int32_t i = 0; i < n; ++i) {
         if (!members[i] || !members[i]->initialized) return MPF_ERR_STATE;
+        if (members[i]->reg) return MPF_ERR_UNSUPPORTED;  // members step without interface regularization
         for (int32_t j = 0; j < i; ++j)
             if (members[j] == members[i]) return MPF_ERR_ARG;
     }
@@ -1720,6 +1799,7 @@ mpf_status mpf_step_into(mpf_ctx* c, double dt, const void* count_in, const void
         return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: " + std::to_string(c->ncells) + " cells exceed its scratch limit of " +
                                                 std::to_string(kScratchCells) + " cells per call (use mpf_step)");
     if (c->ngrains > kScratchGrains) return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: too many grains");
+    if (c->reg) return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: interface regularization needs mpf_step");
     if (avg_active(c))
         return c->fail(MPF_ERR_UNSUPPORTED, "mpf_step_into: driving-force averaging keeps per-pair arrays of the context (use mpf_step)");
     unsigned char* base = nullptr;

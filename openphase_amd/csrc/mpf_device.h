@@ -47,6 +47,16 @@ struct StepArgs {
     // per pair of own entries a < b at plane b(b-1)/2 + a, [pairs][ncells]; null: the phase-pair
     // value (SkipAverage)
     const double* dg_avg;
+    // InterfaceRegularization (DoubleObstacle's curvature overload, in the full kernel variant):
+    // reg = 1 scales a pair's increment left to right, loc_increment * mobility * norm_1 * scale;
+    // reg_active (RegularizationFactor or CurvatureFactor != 1) then multiplies it by reg_factor
+    // and, on interface cells, adds the curvature averaged over the interface (`curv`, one value
+    // per pair of own entries, same layout as dg_avg) * mobility * sqrt(phi_a phi_b) * Pi/Eta;
+    // curv is null when the curvature is not averaged (its merge weight is then 0)
+    int32_t reg;
+    int32_t reg_active;
+    double reg_factor;
+    const double* curv;
     const int32_t* gphase;       // per grain: thermodynamic phase
     const double* gvr;           // per grain: VolumeRatio
     const double* grot;          // per grain: rotation matrix, row-major
@@ -147,6 +157,23 @@ struct AvgArgs {
 };
 constexpr int kAvgBlock = 256;
 int launch_dg_average(const AvgArgs& a, bool fp32, void* stream);
+
+// Interface regularization (mpf_dgavg.hip), FP64 state: the raw curvature of every own pair of
+// the interface cells into `raw`, then InterfaceRegularization's CollectAverage into `tmp` and
+// DistributeAverage into `avg` (= s.curv), all [pairs][ncells] like AvgArgs.
+struct CurvArgs {
+    StepArgs s;                  // state of the step start, grains, tables
+    double* raw;
+    double* tmp;
+    double* avg;
+    const int32_t* off;          // sphere offsets within Range (3 ints each), x outer, z inner
+    const double* off_w;         // Range - |offset|
+    int32_t noff;
+    double threshold;            // sqrt(PhiThreshold * (1 - PhiThreshold))
+    double factor;               // CurvatureFactor - RegularizationFactor
+    double prefactor2;           // Eta / Pi
+};
+int launch_curvature(const CurvArgs& a, bool average, void* stream);
 
 // Host-side launchers (mpf_kernels.hip); return a hipError_t value. The step, deferred-cell and
 // grain kernels run over one or several domains of the same kernel variant in one launch: `args`

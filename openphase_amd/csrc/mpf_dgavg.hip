@@ -176,7 +176,166 @@ __global__ __launch_bounds__(kAvgBlock) void dg_average_kernel(const AvgArgs P)
     }
 }
 
+// ---- interface regularization (FP64 state) -------------------------------------------------
+
+constexpr int kCurvBlock = 256;
+
+// The curvature DoubleObstacle's regularization overload adds per pair of non-zero entries of a
+// wide-interface cell (src/DoubleObstacle.cpp:702-763): loc_increment (the pair's increment before
+// the mobility, over the neighbour-merged list) * Eta/Pi * scale * norm_1 / sqrt(phi_a phi_b) *
+// (CurvatureFactor - RegularizationFactor). Only own entries are non-zero, so only interface cells
+// (more than one own entry) hold such pairs. One thread per cell; the list is gathered as the
+// step's deferred path does, energies per pair from the phase-pair table or the orientation.
+__global__ __launch_bounds__(kCurvBlock) void curvature_raw_kernel(const CurvArgs P)
+{
+    constexpr int K = kWideSlots;
+    const StepArgs& A = P.s;
+    const int64_t N = A.ncells;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kCurvBlock + threadIdx.x;
+    if (c >= N || ListCode<double>::count(A.cnt_in[c]) < 2) return;
+    CellList<K, double, true> L;
+    for (int t = 0; t < K; ++t) L.eid[t] = 0u;
+    // more than K merged entries: the step itself reports the overflow
+    if (gather_cell<K, double, double, true>(A, c, L) != kCellWide) return;
+    const int ne = L.ne, n0 = L.n0, NP = A.nphases;
+    double localn = 0.0;
+    if (A.consider_nucleus) {
+        for (int t = 0; t < ne; ++t) localn += A.gvr[L.eid[t]];
+    } else {
+        localn = static_cast<double>(ne);
+    }
+    const double norm_1 = 1.0 / localn;
+    const double Pf = A.prefactor;
+    // InterfaceProperties energy of a pair (the lower list position first)
+    auto sigma_of = [&](int x, int y) -> double {
+        const int lo = x < y ? x : y, hi = x < y ? y : x;
+        const int k = A.gphase[L.eid[lo]] * NP + A.gphase[L.eid[hi]];
+        return ip_pair_energy<double>(A, k, L.eid[lo], L.eid[hi], L.phi[lo], L.phi[hi], L.gr[lo], L.gr[hi]);
+    };
+    for (int b = 1; b < n0; ++b)
+        for (int a = 0; a < b; ++a) {
+            const double scale = sqrt(A.gvr[L.eid[a]] * A.gvr[L.eid[b]]);
+            const double cf = 0.5 * (1.0 + scale) * Pf;
+            double d = sigma_of(a, b) * ((L.lap[a] + cf * L.phi[a]) - (L.lap[b] + cf * L.phi[b]));
+            if (ne > 2) {
+                const int pa = A.gphase[L.eid[a]], pb = A.gphase[L.eid[b]];
+                for (int g = 0; g < ne; ++g) {
+                    if (g == a || g == b) continue;
+                    d += (sigma_of(b, g) - sigma_of(a, g)) * (L.lap[g] + cf * L.phi[g]);
+                    const int pg = A.gphase[L.eid[g]];
+                    if (A.tj_model == 0) {
+                        if (A.tj_factor != 0.0 || scale < 1.0) {
+                            const double e3 = A.sigma[pa * NP + pb] + A.sigma[pb * NP + pg] + A.sigma[pa * NP + pg];
+                            d += smax(A.tj_factor, (1.0 - scale)) * Pf * e3 * (L.phi[a] * L.phi[g] - L.phi[b] * L.phi[g]);
+                        }
+                    } else if (A.tj_model == 1) {
+                        d += A.tj_energy * Pf * (L.phi[a] * L.phi[g] - L.phi[b] * L.phi[g]);
+                    }
+                }
+            }
+            double kappa = d * P.prefactor2 * scale * norm_1 / sqrt(L.phi[a] * L.phi[b]);
+            kappa *= P.factor;
+            P.raw[static_cast<int64_t>(pair_plane(a, b)) * N + c] = kappa;
+        }
+}
+
+// SECOND = false: InterfaceRegularization::CollectAverageSR (src/InterfaceRegularization.cpp:196-223)
+// into tmp; true: DistributeAverageSR (:225-274) into avg. The pair's weight is sqrt(phi_a phi_b)
+// (SetWeightsSR, :178-194) on interface cells. Collect: for a weight above the threshold, the mean
+// of the neighbours' raw weighted by weight * (Range - |offset|), over terms above DBL_EPSILON, if
+// their sum exceeds DBL_EPSILON (else the fresh entry's 0). Distribute: for a weight above
+// DBL_EPSILON, the plain mean of the tmp of the neighbours whose weight exceeds the threshold, or
+// the own tmp. No grain is a Seed here (grains enter as Stable; only nucleation makes seeds), so
+// the entries below those bounds keep 0. Lane-parallel over the sphere, summed in offset order as
+// dg_average_kernel does.
+template <bool SECOND>
+__global__ __launch_bounds__(kAvgBlock) void curvature_average_kernel(const CurvArgs P)
+{
+    __shared__ uint16_t s_list[kAvgCells];
+    __shared__ int s_count;
+    const StepArgs& A = P.s;
+    const int64_t N = A.ncells;
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kAvgCells;
+    if (wave == 0) {
+        const int64_t c0 = base + lane;
+        const bool inter = lane < kAvgCells && c0 < N && ListCode<double>::count(A.cnt_in[c0]) >= 2;
+        const unsigned long long m = __ballot(inter);
+        if (inter) s_list[__popcll(m & ((1ull << lane) - 1ull))] = static_cast<uint16_t>(lane);
+        if (lane == 0) s_count = __popcll(m);
+    }
+    __syncthreads();
+    const int ninter = s_count;
+    const int64_t nyz = static_cast<int64_t>(A.ny) * A.nz;
+    const double* src = SECOND ? P.tmp : P.raw;
+    for (int i = wave; i < ninter; i += kAvgWaves) {
+        const int64_t c = base + s_list[i];
+        const uint8_t code = A.cnt_in[c];
+        const int n = ListCode<double>::count(code);
+        const int x = static_cast<int>(c / nyz), y = static_cast<int>((c / A.nz) % A.ny), z = static_cast<int>(c % A.nz);
+        for (int b = 1; b < n; ++b)
+            for (int a = 0; a < b; ++a) {
+                const uint32_t ga = A.id_in[static_cast<int64_t>(a) * N + c], gb = A.id_in[static_cast<int64_t>(b) * N + c];
+                const int plane = pair_plane(a, b);
+                const double w = sqrt(entry_value<double>(A, c, code, a) * entry_value<double>(A, c, code, b));
+                double result = 0.0;
+                if (SECOND ? (w > DBL_EPSILON) : (w > P.threshold)) {
+                    double value = 0.0, sum = 0.0;
+                    int counter = 0;
+                    for (int t0 = 0; t0 < P.noff; t0 += 64) {
+                        const int t = t0 + lane;
+                        bool ok = false;
+                        double wt = 0.0, term = 0.0;
+                        if (t < P.noff) {
+                            const int ii = P.off[3 * t], jj = P.off[3 * t + 1], kk = P.off[3 * t + 2];
+                            const int64_t nb = (static_cast<int64_t>(wrap1(x + ii, A.nx)) * A.ny + wrap1(y + jj, A.ny)) * A.nz +
+                                               wrap1(z + kk, A.nz);
+                            double wphi = 0.0;
+                            int np = 0;
+                            bool fwd = true;
+                            if (find_pair<double>(A, nb, ga, gb, wphi, np, fwd)) {
+                                const double v = src[static_cast<int64_t>(np) * N + nb];
+                                if (SECOND) {
+                                    ok = wphi > P.threshold;
+                                    term = fwd ? v : -v;
+                                } else {
+                                    wt = wphi * P.off_w[t];
+                                    ok = wt > DBL_EPSILON;
+                                    term = wt * (fwd ? v : -v);
+                                }
+                            }
+                        }
+                        unsigned long long okm = __ballot(ok);
+                        while (okm) {
+                            const int l = __ffsll(static_cast<long long>(okm)) - 1;
+                            okm &= okm - 1ull;
+                            if (SECOND) counter++;
+                            else sum += lane_value(wt, l);
+                            value += lane_value(term, l);
+                        }
+                    }
+                    if (SECOND) result = counter != 0 ? value / static_cast<double>(counter) : P.tmp[static_cast<int64_t>(plane) * N + c];
+                    else result = sum > DBL_EPSILON ? value / sum : 0.0;
+                }
+                if (lane == 0) (SECOND ? P.avg : P.tmp)[static_cast<int64_t>(plane) * N + c] = result;
+            }
+    }
+}
+
 }  // namespace
+
+int launch_curvature(const CurvArgs& a, bool average, void* stream)
+{
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 raw_grid(static_cast<unsigned>((a.s.ncells + kCurvBlock - 1) / kCurvBlock));
+    hipLaunchKernelGGL(curvature_raw_kernel, raw_grid, dim3(kCurvBlock), 0, s, a);
+    if (average) {
+        const dim3 grid(static_cast<unsigned>((a.s.ncells + kAvgCells - 1) / kAvgCells));
+        hipLaunchKernelGGL((curvature_average_kernel<false>), grid, dim3(kAvgBlock), 0, s, a);
+        hipLaunchKernelGGL((curvature_average_kernel<true>), grid, dim3(kAvgBlock), 0, s, a);
+    }
+    return static_cast<int>(hipGetLastError());
+}
 
 int launch_dg_average(const AvgArgs& a, bool fp32, void* stream)
 {

@@ -144,7 +144,21 @@ __device__ __forceinline__ bool compute_cell(const StepArgs& A, CellList<K, TC, 
             const TC mob = static_cast<TC>(A.mu[ph[a] * NP + ph[b]]);
             L.emax = smax(L.emax, static_cast<double>(sigma_of(a, b)));
             L.mmax = smax(L.mmax, static_cast<double>(mob));
-            d *= mob * static_cast<TC>(norm_1) * scale;
+            if (DG && A.reg) {
+                // DoubleObstacle's curvature overload (src/DoubleObstacle.cpp:753-767), then
+                // InterfaceRegularization::MergePhaseFieldIncrementsSR (src/InterfaceRegularization.cpp:425-456)
+                // on interface cells for the pairs of own entries (non-zero values)
+                d = d * mob * static_cast<TC>(norm_1) * scale;
+                if (A.reg_active) {
+                    d *= static_cast<TC>(A.reg_factor);
+                    if (A.curv != nullptr && n0 > 1 && b < n0) {
+                        const TC kappa = static_cast<TC>(A.curv[static_cast<int64_t>(b * (b - 1) / 2 + a) * A.ncells + cell]);
+                        d += kappa * mob * sqrt(L.ev(a) * L.ev(b)) * static_cast<TC>(A.dg_prefactor);
+                    }
+                }
+            } else {
+                d *= mob * static_cast<TC>(norm_1) * scale;
+            }
             // constant driving force on interface cells (flag 2 = more than one own entry)
             if (DG && n0 > 1 && A.dg_mode[ph[a] * NP + ph[b]] == 1) {
                 const TC raw = dg_local<TC>(A, a, b, n0, cell, ph[a] * NP + ph[b]);
