@@ -405,6 +405,43 @@ def run_dg_average(steps: int):
     return out
 
 
+def run_regularization(steps: int, lists3d):
+    """Interface regularization (DoubleObstacle's curvature overload + InterfaceRegularization
+    Average / MergePhaseFieldIncrements, RegularizationFactor 1.5, CurvatureFactor 0.7, default
+    Range 3 / Threshold) against the plain step, FP64: 2D 512^2 with 200 grains and the 256^3 /
+    1000-grain microstructure of the main leg. Cell-updates/s and ms per step from the wall time of
+    `steps` steps."""
+    from openphase_amd.engine import Engine, StepConfig
+    from openphase_amd.microstructure import voronoi_grains
+
+    out = {"steps": steps,
+           "workload": "regularized vs plain FP64 step (RegularizationFactor 1.5, CurvatureFactor 0.7, Range 3, "
+                       "curvature averaged): 2D 512^2 / 200 grains and 3D 256^3 / 1000 grains, periodic, slots 16 "
+                       "(the regularized 256^3 run outgrows 8 slots within 43 steps)"}
+    for tag, (nx, ny, nz, ng, lists) in (("2d_512", (512, 512, 0, 200, None)), ("3d_256", (256, 256, 256, 1000, lists3d))):
+        if lists is None:
+            lists = voronoi_grains(nx, ny, nz, ng, seed=0x0FE5)
+        ncells = nx * ny * max(nz, 1)
+        for reg in (False, True):
+            cfg = StepConfig(nx, ny, nz, slots=16, regularization=reg, regularization_factor=1.5 if reg else 1.0,
+                             curvature_factor=0.7 if reg else 1.0)
+            dt = 0.5 * cfg.max_time_step()
+            e = Engine(cfg)
+            e.set_grains(np.zeros(ng, np.int32))
+            e.upload(lists)
+            e.finalize_initialization()
+            e.step(dt, 3)
+            _device_sync()
+            t0 = time.perf_counter()
+            e.step(dt, steps)
+            _device_sync()
+            t = time.perf_counter() - t0
+            e.close()
+            out[f"{tag}_{'regularized' if reg else 'plain'}_cell_updates_per_s"] = ncells * steps / t
+            out[f"{tag}_{'regularized' if reg else 'plain'}_ms_per_step"] = 1e3 * t / steps
+    return out
+
+
 def _cpu_model() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -465,6 +502,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=500, help="rollout length of the adjoint leg")
     ap.add_argument("--ensemble-steps", type=int, default=200, help="steps of the ensemble leg")
     ap.add_argument("--dg-steps", type=int, default=200, help="steps of the driving-force averaging leg")
+    ap.add_argument("--reg-steps", type=int, default=40, help="steps of the interface regularization leg")
+    ap.add_argument("--only-regularization", action="store_true",
+                    help="only the interface regularization leg (for profiling runs); prints its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the step on a single MI355X "
@@ -481,13 +521,16 @@ def main():
     t0 = time.perf_counter()
     lists = voronoi_grains(NX, NY, NZ, NGRAINS, seed=SEED, sides="plus")
     print(f"[bench] microstructure {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    if args.only_regularization:
+        print(json.dumps(run_regularization(args.reg_steps, lists)))
+        return
     r64 = run_gpu(0, lists, args.steps, args.warmup)
     print(f"[bench] fp64 {r64}", file=sys.stderr, flush=True)
     r32 = run_gpu(1, lists, args.steps, args.warmup)
     print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
     acc32 = None if args.no_fp32_accuracy else run_fp32_accuracy(lists)
     print(f"[bench] fp32 accuracy {acc32}", file=sys.stderr, flush=True)
-    adj = ens = dga = cpu = None
+    adj = ens = dga = cpu = reg = None
     if not args.only_main:
         adj = {"fp32": run_adjoint(args.adjoint_steps, 50, 1)}
         print(f"[bench] adjoint fp32 {adj['fp32']}", file=sys.stderr, flush=True)
@@ -497,6 +540,8 @@ def main():
         print(f"[bench] ensemble {ens}", file=sys.stderr, flush=True)
         dga = run_dg_average(args.dg_steps)
         print(f"[bench] driving-force averaging {dga}", file=sys.stderr, flush=True)
+        reg = run_regularization(args.reg_steps, lists)
+        print(f"[bench] interface regularization {reg}", file=sys.stderr, flush=True)
     if not (args.no_cpu or args.only_main):
         cpu = cpu_baseline(lists)
     print(f"[bench] cpu {cpu}", file=sys.stderr, flush=True)
@@ -564,6 +609,7 @@ def main():
         "adjoint": adj,
         "ensemble": ens,
         "dg_average": dga,
+        "regularization": reg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

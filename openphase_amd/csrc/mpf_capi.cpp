@@ -376,6 +376,7 @@ mpf::CurvArgs curv_args(const mpf_ctx* c, const mpf::StepArgs& s)
     a.threshold = std::sqrt(c->reg_threshold * (1.0 - c->reg_threshold));
     a.factor = c->curvature_factor - c->regularization;
     a.prefactor2 = c->grid.iwidth * c->grid.dx / kPi;
+    a.grad = c->grad ? 1 : 0;
     return a;
 }
 

@@ -172,6 +172,7 @@ struct CurvArgs {
     double threshold;            // sqrt(PhiThreshold * (1 - PhiThreshold))
     double factor;               // CurvatureFactor - RegularizationFactor
     double prefactor2;           // Eta / Pi
+    int32_t grad;                // orientation-dependent energies present (gradients needed)
 };
 int launch_curvature(const CurvArgs& a, bool average, void* stream);
 

@@ -184,23 +184,33 @@ constexpr int kCurvBlock = 256;
 // wide-interface cell (src/DoubleObstacle.cpp:702-763): loc_increment (the pair's increment before
 // the mobility, over the neighbour-merged list) * Eta/Pi * scale * norm_1 / sqrt(phi_a phi_b) *
 // (CurvatureFactor - RegularizationFactor). Only own entries are non-zero, so only interface cells
-// (more than one own entry) hold such pairs. One thread per cell; the list is gathered as the
-// step's deferred path does, energies per pair from the phase-pair table or the orientation.
-__global__ __launch_bounds__(kCurvBlock) void curvature_raw_kernel(const CurvArgs P)
+// (more than one own entry) hold such pairs. The list of up to K merged entries is gathered as the
+// step's deferred path does and kept in registers (loops fully unrolled over K, guarded by the
+// counts); energies per pair from the phase-pair table (GRAD false) or the orientation. Returns
+// false when the cell has more than K merged entries.
+template <int K, bool GRAD>
+__device__ __forceinline__ bool raw_cell(const CurvArgs& P, int64_t c)
 {
-    constexpr int K = kWideSlots;
     const StepArgs& A = P.s;
     const int64_t N = A.ncells;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kCurvBlock + threadIdx.x;
-    if (c >= N || ListCode<double>::count(A.cnt_in[c]) < 2) return;
-    CellList<K, double, true> L;
+    CellList<K, double, GRAD> L;
+#pragma unroll
     for (int t = 0; t < K; ++t) L.eid[t] = 0u;
-    // more than K merged entries: the step itself reports the overflow
-    if (gather_cell<K, double, double, true>(A, c, L) != kCellWide) return;
+    const int r = gather_cell<K, double, double, GRAD>(A, c, L);
+    if (r == kCellDeferred) return false;
+    if (r != kCellWide) return true;
     const int ne = L.ne, n0 = L.n0, NP = A.nphases;
+    int ph[K];
+    double vr[K];
     double localn = 0.0;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        ph[t] = 0; vr[t] = 0.0;
+        if (t < ne) { ph[t] = A.gphase[L.eid[t]]; vr[t] = A.gvr[L.eid[t]]; }
+    }
     if (A.consider_nucleus) {
-        for (int t = 0; t < ne; ++t) localn += A.gvr[L.eid[t]];
+#pragma unroll
+        for (int t = 0; t < K; ++t) if (t < ne) localn += vr[t];
     } else {
         localn = static_cast<double>(ne);
     }
@@ -209,23 +219,27 @@ __global__ __launch_bounds__(kCurvBlock) void curvature_raw_kernel(const CurvArg
     // InterfaceProperties energy of a pair (the lower list position first)
     auto sigma_of = [&](int x, int y) -> double {
         const int lo = x < y ? x : y, hi = x < y ? y : x;
-        const int k = A.gphase[L.eid[lo]] * NP + A.gphase[L.eid[hi]];
-        return ip_pair_energy<double>(A, k, L.eid[lo], L.eid[hi], L.phi[lo], L.phi[hi], L.gr[lo], L.gr[hi]);
+        const int k = ph[lo] * NP + ph[hi];
+        if (!GRAD) return A.sigma[k];
+        return ip_pair_energy<double>(A, k, L.eid[lo], L.eid[hi], L.phi[lo], L.phi[hi], L.gr[GRAD ? lo : 0],
+                                      L.gr[GRAD ? hi : 0]);
     };
-    for (int b = 1; b < n0; ++b)
+#pragma unroll
+    for (int b = 1; b < K; ++b)
+#pragma unroll
         for (int a = 0; a < b; ++a) {
-            const double scale = sqrt(A.gvr[L.eid[a]] * A.gvr[L.eid[b]]);
+            if (b >= n0) continue;
+            const double scale = sqrt(vr[a] * vr[b]);
             const double cf = 0.5 * (1.0 + scale) * Pf;
             double d = sigma_of(a, b) * ((L.lap[a] + cf * L.phi[a]) - (L.lap[b] + cf * L.phi[b]));
             if (ne > 2) {
-                const int pa = A.gphase[L.eid[a]], pb = A.gphase[L.eid[b]];
-                for (int g = 0; g < ne; ++g) {
-                    if (g == a || g == b) continue;
+#pragma unroll
+                for (int g = 0; g < K; ++g) {
+                    if (g == a || g == b || g >= ne) continue;
                     d += (sigma_of(b, g) - sigma_of(a, g)) * (L.lap[g] + cf * L.phi[g]);
-                    const int pg = A.gphase[L.eid[g]];
                     if (A.tj_model == 0) {
                         if (A.tj_factor != 0.0 || scale < 1.0) {
-                            const double e3 = A.sigma[pa * NP + pb] + A.sigma[pb * NP + pg] + A.sigma[pa * NP + pg];
+                            const double e3 = A.sigma[ph[a] * NP + ph[b]] + A.sigma[ph[b] * NP + ph[g]] + A.sigma[ph[a] * NP + ph[g]];
                             d += smax(A.tj_factor, (1.0 - scale)) * Pf * e3 * (L.phi[a] * L.phi[g] - L.phi[b] * L.phi[g]);
                         }
                     } else if (A.tj_model == 1) {
@@ -237,6 +251,17 @@ __global__ __launch_bounds__(kCurvBlock) void curvature_raw_kernel(const CurvArg
             kappa *= P.factor;
             P.raw[static_cast<int64_t>(pair_plane(a, b)) * N + c] = kappa;
         }
+    return true;
+}
+
+// One thread per cell: the lists of up to 8 merged entries in registers, longer ones (more than
+// 16: the step itself reports the overflow) with 16.
+template <bool GRAD>
+__global__ __launch_bounds__(kCurvBlock) void curvature_raw_kernel(const CurvArgs P)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kCurvBlock + threadIdx.x;
+    if (c >= P.s.ncells || ListCode<double>::count(P.s.cnt_in[c]) < 2) return;
+    if (!raw_cell<8, GRAD>(P, c)) (void)raw_cell<kWideSlots, GRAD>(P, c);
 }
 
 // SECOND = false: InterfaceRegularization::CollectAverageSR (src/InterfaceRegularization.cpp:196-223)
@@ -328,7 +353,8 @@ int launch_curvature(const CurvArgs& a, bool average, void* stream)
 {
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 raw_grid(static_cast<unsigned>((a.s.ncells + kCurvBlock - 1) / kCurvBlock));
-    hipLaunchKernelGGL(curvature_raw_kernel, raw_grid, dim3(kCurvBlock), 0, s, a);
+    if (a.grad) hipLaunchKernelGGL((curvature_raw_kernel<true>), raw_grid, dim3(kCurvBlock), 0, s, a);
+    else hipLaunchKernelGGL((curvature_raw_kernel<false>), raw_grid, dim3(kCurvBlock), 0, s, a);
     if (average) {
         const dim3 grid(static_cast<unsigned>((a.s.ncells + kAvgCells - 1) / kAvgCells));
         hipLaunchKernelGGL((curvature_average_kernel<false>), grid, dim3(kAvgBlock), 0, s, a);
