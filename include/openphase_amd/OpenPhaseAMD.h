@@ -55,6 +55,7 @@ class InputFile {
             if (line[p] == '@') {
                 std::istringstream ss(line.substr(p + 1));
                 ss >> module;
+                entries_[module];  // a module without keys still counts as present
                 continue;
             }
             if (line[p] != '$') continue;
@@ -70,6 +71,7 @@ class InputFile {
             if (!entries_[module].count(key)) entries_[module][key] = value;
         }
     }
+    bool has_module(const std::string& module) const { return entries_.count(module) != 0; }
     bool has(const std::string& module, const std::string& key) const
     {
         auto m = entries_.find(module);
@@ -412,9 +414,15 @@ class PhaseField {
         // and DrivingForce::Average)
         if (driving_force_)
             check(mpf_set_step_terms(ctx_, dg_step_ != 0, dg_step_ == 2), ctx_, "PhaseField::MergeIncrements");
+        // the averaged curvature is merged on the device: the driver must have asked for both
+        // (without Average the reference merges it with weight 0, i.e. not at all)
+        if (regularization_ && reg_average_ && reg_calls_ != 7)
+            throw std::logic_error("a regularized step needs InterfaceRegularization::Average and "
+                                   "MergePhaseFieldIncrements after CalculatePhaseFieldIncrements(Phase, IP, Kappa)");
         check(mpf_step(ctx_, dt, 1), ctx_, "PhaseField::MergeIncrements");
         stage_ = 0;
         dg_step_ = 0;
+        reg_calls_ = 0;
         ++steps_;
     }
     // Runs n steps of the recorded sequence at once (one synchronisation): the calls of the last
@@ -445,6 +453,7 @@ class PhaseField {
     friend class DoubleObstacle;
     friend class DrivingForce;
     friend class UserDrivingForce;
+    friend class InterfaceRegularization;
 
     void check_bc(const BoundaryConditions& BC)
     {
@@ -466,6 +475,9 @@ class PhaseField {
     double dt_ = 0.0;
     bool driving_force_ = false;  // the input file gives a constant driving force (device tables)
     int dg_step_ = 0;    // driving force merged into this step: 0 none, 1 raw (SkipAverage), 2 averaged
+    bool regularization_ = false;  // the input file has @InterfaceRegularization: every step takes its overload
+    bool reg_average_ = true;      // its Average setting
+    int reg_calls_ = 0;  // this step: 1 regularized increments, |2 Kappa.Average, |4 Kappa.MergePhaseFieldIncrements
     long steps_ = 0;     // steps run on the device
     long ip_step_ = -1;  // steps_ at the last IP.Set on this phase field (-1: none)
     std::vector<double> fractions_;  // Fractions downloaded at step fractions_step_
@@ -506,6 +518,7 @@ class InterfaceProperties {
         else if (tj == "NONE") TripleJunctionModel = MPF_TJ_NONE;
         else throw std::runtime_error("unknown TripleJunctionModel " + tj);
         RegularizationFactor = in.getd("InterfaceProperties", "RegularizationFactor", 1.0);
+        CurvatureFactor = in.getd("InterfaceProperties", "CurvatureFactor", 1.0);
     }
     // InterfaceProperties::Set (src/InterfaceProperties.cpp:557-576): first call of a step
     void Set(PhaseField& Phase, const BoundaryConditions& BC)
@@ -534,6 +547,7 @@ class InterfaceProperties {
     double TripleJunctionFactor = 0.0;
     double TripleJunctionEnergy = 0.0;
     double RegularizationFactor = 1.0;
+    double CurvatureFactor = 1.0;
 
  private:
     friend class PhaseField;
@@ -632,6 +646,46 @@ inline void UserDrivingForce::SetDrivingForce(PhaseField& Phase, DrivingForce& d
 }
 
 // ------------------------------------------------------------------------------------------
+// InterfaceRegularization (include/InterfaceRegularization.h): the curvature storage lives on the
+// device and is never materialised on the host; these calls record which steps of the module the
+// driver asked for (the fused step runs them in PhaseField::MergeIncrements).
+class InterfaceRegularization {
+ public:
+    // InterfaceRegularization::Initialize + ReadInput (src/InterfaceRegularization.cpp:46-118):
+    // Average (default Yes), Range ((IWidth + 1)/2), Threshold (DBL_EPSILON), ClearingMode
+    InterfaceRegularization(Settings& s, const std::string& file) { Read(InputFile(file), s.Grid.iWidth); }
+    void Read(const InputFile& in, double iwidth)
+    {
+        Averaging = in.getb("InterfaceRegularization", "Average", true);
+        Range = in.geti("InterfaceRegularization", "Range", static_cast<int>((iwidth + 1) / 2));
+        PhiThreshold = in.getd("InterfaceRegularization", "Threshold", DBL_EPSILON);
+        if (in.get("InterfaceRegularization", "ClearingMode", "AUTOMATIC") != "AUTOMATIC")
+            throw std::invalid_argument("InterfaceRegularization: only ClearingMode AUTOMATIC is supported");
+    }
+    // InterfaceRegularization::Clear: the device storage is cleared at every step
+    void Clear() {}
+    // InterfaceRegularization::Average (src/InterfaceRegularization.cpp:156-176)
+    void Average(const PhaseField& Phase, const BoundaryConditions&)
+    {
+        if (Phase.stage_ != 2 || !(Phase.reg_calls_ & 1))
+            throw std::logic_error("InterfaceRegularization::Average must follow "
+                                   "DoubleObstacle::CalculatePhaseFieldIncrements(Phase, IP, Kappa)");
+        const_cast<PhaseField&>(Phase).reg_calls_ |= 2;
+    }
+    // InterfaceRegularization::MergePhaseFieldIncrements (src/InterfaceRegularization.cpp:403-456)
+    void MergePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties&)
+    {
+        if (Phase.stage_ != 2 || !(Phase.reg_calls_ & 1))
+            throw std::logic_error("InterfaceRegularization::MergePhaseFieldIncrements must follow the regularized "
+                                   "increments and precede NormalizeIncrements");
+        Phase.reg_calls_ |= 4;
+    }
+    bool Averaging = true;
+    int Range = 3;
+    double PhiThreshold = DBL_EPSILON;
+};
+
+// ------------------------------------------------------------------------------------------
 // DoubleObstacle (include/DoubleObstacle.h)
 class DoubleObstacle {
  public:
@@ -641,6 +695,9 @@ class DoubleObstacle {
     void CalculatePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties&)
     {
         if (Phase.stage_ != 1) throw std::logic_error("CalculatePhaseFieldIncrements before IP.Set");
+        if (Phase.regularization_)
+            throw std::logic_error("the input file sets up InterfaceRegularization: call "
+                                   "CalculatePhaseFieldIncrements(Phase, IP[, dG], Kappa)");
         Phase.stage_ = 2;
     }
     // The overload with the driving force computes the same increments: the force acts through
@@ -648,6 +705,24 @@ class DoubleObstacle {
     void CalculatePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties& IP, DrivingForce&)
     {
         CalculatePhaseFieldIncrements(Phase, IP);
+    }
+    // The curvature overloads (src/DoubleObstacle.cpp:644-684, 686-770): increments times
+    // RegularizationFactor plus the curvature InterfaceRegularization averages and merges; they
+    // need the @InterfaceRegularization block in the phase field's input file (the device context
+    // is set up for it)
+    void CalculatePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties&, InterfaceRegularization&)
+    {
+        if (Phase.stage_ != 1) throw std::logic_error("CalculatePhaseFieldIncrements before IP.Set");
+        if (!Phase.regularization_)
+            throw std::logic_error("CalculatePhaseFieldIncrements(Phase, IP, Kappa): the phase field's input file has "
+                                   "no @InterfaceRegularization block");
+        Phase.stage_ = 2;
+        Phase.reg_calls_ = 1;
+    }
+    void CalculatePhaseFieldIncrements(PhaseField& Phase, InterfaceProperties& IP, DrivingForce&,
+                                       InterfaceRegularization& Kappa)
+    {
+        CalculatePhaseFieldIncrements(Phase, IP, Kappa);
     }
     // DoubleObstacle::AverageEnergyDensity (src/DoubleObstacle.cpp:989-1005): PointEnergy summed
     // over the grid / number of cells, with the interface energies of IP's last Set on Phase
@@ -709,6 +784,16 @@ inline void PhaseField::ensure_context()
             p.dg_weights = df.WeightsMode;
         }
         p.regularization_factor = ip.RegularizationFactor;
+        if (InputFile(file_).has_module("InterfaceRegularization")) {
+            InterfaceRegularization kappa(settings_, file_);
+            p.regularization = 1;
+            p.curvature_factor = ip.CurvatureFactor;
+            p.reg_average = kappa.Averaging ? 1 : 0;
+            p.reg_range = kappa.Range;
+            p.reg_threshold = kappa.PhiThreshold;
+            regularization_ = true;
+            reg_average_ = kappa.Averaging;
+        }
         check(mpf_create(&g, &p, &ctx_), nullptr, "mpf_create");
         driving_force_ = static_cast<bool>(udf);
     }

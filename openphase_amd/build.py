@@ -71,7 +71,7 @@ def build_hip() -> Path:
 def build_examples() -> None:
     """The C++ drop-in examples (OpenPhase-style classes over the C ABI), linked to libmpf_hip.so."""
     cxx = shutil.which("g++") or "g++"
-    for name in ("normal_gg", "driving_force"):
+    for name in ("normal_gg", "driving_force", "regularization"):
         src = PKG / "examples" / f"{name}.cpp"
         out = PKG / "examples" / name
         _run([cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(src), f"-L{PKG}", "-lmpf_hip",

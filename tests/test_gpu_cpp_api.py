@@ -163,3 +163,54 @@ def test_normal_gg_ignores_an_unapplied_driving_force(tmp_path):
     # the same loop with the force applied ends elsewhere
     with_force, _ = _oracle_run(cfg, lists, phase, dt, steps, 1)
     assert not (np.array_equal(with_force.index, exp.index) and np.abs(with_force.value - exp.value).max() < 1e-9)
+
+
+REG_EXAMPLE = O.HERE.parent / "openphase_amd" / "examples" / "regularization"
+
+
+def test_regularization_loop_matches_oracle(tmp_path):
+    """IP.Set -> DO.CalculatePhaseFieldIncrements(Phi, IP, Kappa) -> Kappa.Average ->
+    Kappa.MergePhaseFieldIncrements -> Normalize -> Merge through the C++ classes, on an input file
+    with an @InterfaceRegularization block (RegularizationFactor 1.5, CurvatureFactor 0.7): the
+    state and the energies equal the oracle's regularized steps."""
+    if not REG_EXAMPLE.exists():
+        pytest.fail("openphase_amd/examples/regularization is not built")
+    steps, ngrains = 20, 12
+    cfg = StepConfig(40, 40, 0, regularization=True, regularization_factor=1.5, curvature_factor=0.7)
+    lists = voronoi_grains(40, 40, 0, ngrains, seed=23)
+    dt = 0.9 * cfg.max_time_step()
+    O.write_opi(tmp_path / "in.opi", cfg, dt=dt, nsteps=steps)
+    rawio.write_phasefield(str(tmp_path / "in.dat"), lists)
+    print(f"regularization example {steps} steps", flush=True)
+    res = subprocess.run([str(REG_EXAMPLE), str(tmp_path / "in.opi"), str(tmp_path / "in.dat"), str(ngrains), str(steps),
+                          str(tmp_path / "out.dat")], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    orc = O.Oracle(cfg)
+    orc.set_grains(np.zeros(ngrains, np.int32))
+    orc.upload(lists)
+    orc.finalize_initialization()
+    orc.keep_step_start(True)
+    energies = []
+    for _ in range(steps):
+        orc.step(dt, 1)
+        energies.append(orc.interface_energy("previous") / lists.ncells)
+    exp = orc.download()
+    full = rawio.read_phasefield(str(tmp_path / "out.dat"))
+    keep = full.value != 0.0
+    cell = np.repeat(np.arange(full.ncells), full.count)
+    assert np.array_equal(np.bincount(cell[keep], minlength=full.ncells), exp.count)
+    assert np.array_equal(full.index[keep], exp.index)
+    assert np.abs(full.value[keep] - exp.value).max() <= 1e-12
+    got = _energies(res.stdout)
+    assert sorted(got) == list(range(steps))
+    for t in range(steps):
+        assert got[t] == pytest.approx(energies[t], rel=1e-12)
+    # the plain step differs: the regularization acted
+    plain = O.Oracle(StepConfig(40, 40, 0))
+    plain.set_grains(np.zeros(ngrains, np.int32))
+    plain.upload(lists)
+    plain.finalize_initialization()
+    plain.step(dt, steps)
+    other = plain.download()
+    assert not (np.array_equal(other.count, exp.count) and np.array_equal(other.index, exp.index)
+                and np.abs(other.value - exp.value).max() < 1e-9)
