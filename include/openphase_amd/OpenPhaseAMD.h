@@ -16,8 +16,8 @@
 // Errors are reported by exceptions (std::runtime_error); nothing calls exit().
 //
 // Input: the same .opi files (keys of @GridParameters, @Settings, @PhaseField,
-// @InterfaceProperties, @BoundaryConditions, @UserDrivingForce, @DrivingForce read by the
-// reference's FileInterface, src/FileInterface.cpp:129-345).
+// @InterfaceProperties, @BoundaryConditions, @UserDrivingForce, @DrivingForce,
+// @InterfaceRegularization read by the reference's FileInterface, src/FileInterface.cpp:129-345).
 #ifndef OPENPHASE_AMD_OPENPHASEAMD_H
 #define OPENPHASE_AMD_OPENPHASEAMD_H
 

@@ -458,7 +458,6 @@ MPF_HD __forceinline__ void cell_adjoint(const AdjointArgs& J, bool fp32, int64_
     const double (&lap)[C] = F.lap;
     const double norm_1 = F.norm_1;
     const double (&sig)[GRAD ? NPC : 1] = F.sig;
-    const double (&psiB)[NPC] = F.psiB;
     const double (&dpre)[NPC] = F.dpre;
     const Part2Rec<C>& R = F.R;
     auto sg = [&](int x, int y) -> double {
