@@ -95,6 +95,12 @@ def cases():
                        dg_mode=np.ones((3, 3), np.int32) - np.eye(3, dtype=np.int32), dg_value=dg3,
                        dg_average=True, regularization=True, regularization_factor=1.2, curvature_factor=0.6)
     out.append(("reg_dg2d_32", regdg, 12, 35, phase3, None, 20, [0, 1, 10, 20], 0.9))
+    # regularization with cubic anisotropy (orientation-dependent pair energies in the curvature)
+    q2, _ = random_rotations(12, 36)
+    regc = StepConfig(32, 32, 0, nphases=2, sigma=np.full((2, 2), 0.24), mu=np.full((2, 2), 4e-9),
+                      energy_model=np.ones((2, 2), np.int32), epsilon=np.full((2, 2), 0.05),
+                      regularization=True, regularization_factor=1.3, curvature_factor=0.9)
+    out.append(("reg_cubic2d_32", regc, 12, 36, phase2, q2, 20, [0, 1, 10, 20], 0.9))
     return out
 
 

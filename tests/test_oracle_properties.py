@@ -279,6 +279,7 @@ def test_averaged_constant_driving_force_is_the_constant():
     ("reg2d_32", {"regularization": False}), ("reg2d_32", {"reg_average": False}), ("reg2d_32", {"reg_range": 2}),
     ("reg3d_16", {"reg_threshold": 0.0}), ("reg_noavg2d_24", {"reg_average": True}),
     ("reg_unit2d_24", {"regularization": False}), ("reg_dg2d_32", {"reg_average": False}),
+    ("reg_cubic2d_32", {"regularization": False}),
 ])
 def test_golden_pins_interface_regularization(name, change):
     """The regularization fixtures separate the curvature overload from the plain step, averaged
