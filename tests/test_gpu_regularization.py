@@ -25,6 +25,9 @@ CASES = {
                       mu=np.array([[4e-9, 2e-9], [2e-9, 3e-9]]), tj_model=1, tj_energy=0.05, slots=16,
                       regularization=True, regularization_factor=2.0, curvature_factor=1.0, reg_range=2,
                       reg_threshold=0.05), 16, 12),
+    # the 256^3 benchmark's grain size on 96^3 (53 grains), 20 steps
+    "3d-96": (StepConfig(96, 96, 96, slots=16, regularization=True, regularization_factor=1.5, curvature_factor=0.7),
+              53, 20),
     # a reduced y axis, curvature not averaged (the merge adds nothing, the increments are scaled)
     "xz-noavg": (StepConfig(40, 0, 36, slots=16, regularization=True, regularization_factor=0.8,
                             curvature_factor=0.5, reg_average=False), 12, 40),
