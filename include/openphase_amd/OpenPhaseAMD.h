@@ -519,6 +519,12 @@ class InterfaceProperties {
         else throw std::runtime_error("unknown TripleJunctionModel " + tj);
         RegularizationFactor = in.getd("InterfaceProperties", "RegularizationFactor", 1.0);
         CurvatureFactor = in.getd("InterfaceProperties", "CurvatureFactor", 1.0);
+        // InterfaceProperties::Extrapolate runs inside Set (src/InterfaceProperties.cpp:575,
+        // 1222-1249); ZEROTHORDER does nothing there, FIRSTORDER rewrites the energies of
+        // interface-halo nodes, which this build does not model
+        const std::string ex = in.get("InterfaceProperties", "ExtrapolationMode", "NONE");
+        if (ex != "NONE" && ex != "ZEROTHORDER")
+            throw std::runtime_error("ExtrapolationMode " + ex + " is not implemented");
     }
     // InterfaceProperties::Set (src/InterfaceProperties.cpp:557-576): first call of a step
     void Set(PhaseField& Phase, const BoundaryConditions& BC)

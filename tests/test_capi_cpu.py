@@ -41,6 +41,58 @@ def test_cpp_interface_compiles():
     assert res.returncode == 0, res.stderr
 
 
+def test_cpp_interface_reads_regularization_and_refuses_extrapolation(tmp_path):
+    """The C++ classes read @InterfaceRegularization and CurvatureFactor from the input file, and
+    refuse ExtrapolationMode FIRSTORDER (the reference rewrites halo energies inside IP.Set) instead
+    of stepping without it; ZEROTHORDER, a no-op in the reference, is accepted. Input parsing only:
+    no device context is created."""
+    import shutil
+    import subprocess
+
+    import oracle as O
+    from openphase_amd.engine import StepConfig
+
+    gxx = shutil.which("g++")
+    if not gxx or not _capi.LIB_PATH.exists():
+        pytest.skip("no g++ or libmpf_hip.so not built")
+    src = tmp_path / "probe.cpp"
+    src.write_text(r"""
+#include <cstdio>
+#include "openphase_amd/OpenPhaseAMD.h"
+using namespace openphase_amd;
+int main(int, char* argv[])
+{
+    try {
+        Settings S; S.ReadInput(argv[1]);
+        InterfaceProperties IP(S, argv[1]);
+        InterfaceRegularization K(S, argv[1]);
+        std::printf("ok %.17g %.17g %d %d %.17g\n", IP.RegularizationFactor, IP.CurvatureFactor, int(K.Averaging),
+                    K.Range, K.PhiThreshold);
+    } catch (const std::exception& e) {
+        std::printf("error %s\n", e.what());
+    }
+    return 0;
+}
+""")
+    exe = tmp_path / "probe"
+    res = subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}", str(src),
+                          f"-L{_capi.LIB_PATH.parent}", "-lmpf_hip", f"-Wl,-rpath,{_capi.LIB_PATH.parent}", "-o", str(exe)],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    cfg = StepConfig(32, 32, 0, regularization=True, regularization_factor=1.5, curvature_factor=0.7, reg_range=2,
+                     reg_threshold=0.05)
+    opi = tmp_path / "in.opi"
+    O.write_opi(opi, cfg)
+    out = subprocess.run([str(exe), str(opi)], capture_output=True, text=True, timeout=60).stdout.split()
+    assert out[0] == "ok" and float(out[1]) == 1.5 and float(out[2]) == 0.7
+    assert out[3:5] == ["1", "2"] and float(out[5]) == 0.05
+    for mode, ok in (("ZEROTHORDER", True), ("FIRSTORDER", False)):
+        text = opi.read_text().replace("@InterfaceProperties\n", f"@InterfaceProperties\n$ExtrapolationMode : {mode}\n")
+        (tmp_path / "ex.opi").write_text(text)
+        got = subprocess.run([str(exe), str(tmp_path / "ex.opi")], capture_output=True, text=True, timeout=60).stdout
+        assert got.startswith("ok") == ok, got
+
+
 def test_invalid_arguments_fail_without_a_gpu():
     """Argument checks run before any device call: a null grid is MPF_ERR_ARG."""
     if not _capi.LIB_PATH.exists():
