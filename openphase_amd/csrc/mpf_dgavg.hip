@@ -254,14 +254,31 @@ __device__ __forceinline__ bool raw_cell(const CurvArgs& P, int64_t c)
     return true;
 }
 
-// One thread per cell: the lists of up to 8 merged entries in registers, longer ones (more than
-// 16: the step itself reports the overflow) with 16.
+constexpr int kRawCells = 4 * kCurvBlock;  // cells per block of the raw pass
+
+// A block takes kRawCells consecutive cells, collects its interface cells in LDS (in any order:
+// each cell's result is its own) and gives one to each thread in turn, so no lane idles on bulk
+// cells. Lists of up to 8 merged entries in registers, longer ones (more than 16: the step
+// itself reports the overflow) with 16.
 template <bool GRAD>
 __global__ __launch_bounds__(kCurvBlock) void curvature_raw_kernel(const CurvArgs P)
 {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kCurvBlock + threadIdx.x;
-    if (c >= P.s.ncells || ListCode<double>::count(P.s.cnt_in[c]) < 2) return;
-    if (!raw_cell<8, GRAD>(P, c)) (void)raw_cell<kWideSlots, GRAD>(P, c);
+    __shared__ uint16_t s_list[kRawCells];
+    __shared__ int s_n;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kRawCells;
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    for (int k = tid; k < kRawCells; k += kCurvBlock) {
+        const int64_t c = base + k;
+        if (c < P.s.ncells && ListCode<double>::count(P.s.cnt_in[c]) >= 2) s_list[atomicAdd(&s_n, 1)] = static_cast<uint16_t>(k);
+    }
+    __syncthreads();
+    const int n = s_n;
+    for (int i = tid; i < n; i += kCurvBlock) {
+        const int64_t c = base + s_list[i];
+        if (!raw_cell<8, GRAD>(P, c)) (void)raw_cell<kWideSlots, GRAD>(P, c);
+    }
 }
 
 // SECOND = false: InterfaceRegularization::CollectAverageSR (src/InterfaceRegularization.cpp:196-223)
@@ -352,7 +369,7 @@ __global__ __launch_bounds__(kAvgBlock) void curvature_average_kernel(const Curv
 int launch_curvature(const CurvArgs& a, bool average, void* stream)
 {
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const dim3 raw_grid(static_cast<unsigned>((a.s.ncells + kCurvBlock - 1) / kCurvBlock));
+    const dim3 raw_grid(static_cast<unsigned>((a.s.ncells + kRawCells - 1) / kRawCells));
     if (a.grad) hipLaunchKernelGGL((curvature_raw_kernel<true>), raw_grid, dim3(kCurvBlock), 0, s, a);
     else hipLaunchKernelGGL((curvature_raw_kernel<false>), raw_grid, dim3(kCurvBlock), 0, s, a);
     if (average) {
