@@ -287,7 +287,9 @@ void tile_shape(const mpf_ctx* c, int tile_x, int* ty, int* tz, int* tx)
 {
     *ty = *tz = *tx = 0;
     if (c->active < 2) return;
-    int z = c->dnz ? std::min(c->nz, c->dny ? 64 : 256) : 1;
+    // 16 x 16 in 3D (the tile the autotuner picks at 256^3: a smaller halo than 4 x 64), 64 cells
+    // along z in 2D (y, z) planes, whole z rows otherwise
+    int z = c->dnz ? std::min(c->nz, c->dny ? (c->dnx ? 16 : 64) : 256) : 1;
     int y = c->dny ? std::min(c->ny, std::max(1, mpf::kBlock / z)) : 1;
     while ((y + 2 * c->dny) * (z + 2 * c->dnz) > mpf::kTileHalo && y > 1) --y;
     while ((y + 2 * c->dny) * (z + 2 * c->dnz) > mpf::kTileHalo && z > 1) --z;
